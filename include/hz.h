@@ -1,0 +1,195 @@
+/*
+ * hz.h -- C ABI of the MI355X-native DirectPseudoSpectral Helmholtz solver.
+ *
+ * Solves  d2u/dx2 + d2u/dy2 + d2u/dz2 - lambda u = f  on a box that is
+ * periodic (Fourier) in x and y and discretised in z by E spectral elements
+ * of N Gauss-Lobatto-Legendre points, with Robin data on the two z faces:
+ *     beta_l u - alpha_l du/dz = g_l   at the first z node,
+ *     beta_r u + alpha_r du/dz = g_r   at the last z node.
+ *
+ * It replaces the C++ interface of gpritam/MPI,
+ * HelmholtzSolvers/DirectPseudoSpectral/include/NonPeriodic3D_MPI.h:33-47
+ * (CreatePDEPlan / DestroyPDEPlan / SolvePDE / FirstDerivativeZ /
+ * SecondDerivativeZ) and LU_Decomposition.h:8-9, with an explicit plan
+ * handle instead of process globals.  The C++ drop-in with the reference's
+ * names and globals (hz_compat.h) is built on top of this header.
+ *
+ * Conventions
+ *  - All arrays passed to hz_solve* / hz_adjoint / hz_derivative_z are DEVICE
+ *    pointers (HIP), float64, row-major with z fastest:
+ *        f, u          [nx][ny][nz_total],   nz_total = ne*(n-1)+1
+ *        g_l, g_r      [nx][ny]
+ *  - Every entry point returns 0 on success or a positive HZ_ERR_* code;
+ *    hz_last_error_string() describes the last failure of the calling thread.
+ *  - Work is enqueued on the caller's stream (NULL = default stream); the
+ *    call does not synchronise unless the entry point says so.
+ *  - A plan is not thread-safe: concurrent calls on one plan race on its
+ *    workspace.  Different plans are independent.
+ */
+#ifndef HZ_H
+#define HZ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hz_plan hz_plan;
+typedef struct ihipStream_t *hz_stream_t; /* == hipStream_t */
+
+enum {
+    HZ_OK = 0,
+    HZ_ERR_ARG = 1,          /* invalid argument or descriptor */
+    HZ_ERR_NOT_POW2 = 2,     /* Fourier_Quadrature.cpp:1106-1114 */
+    HZ_ERR_N_TOO_SMALL = 3,  /* GLLQuadrature.cpp:82-83 */
+    HZ_ERR_SINGULAR = 4,     /* LU_Decomposition.cpp:49-50 */
+    HZ_ERR_ALLOC = 5,        /* General.h:311-312 */
+    HZ_ERR_HIP = 6,          /* a HIP runtime call failed */
+    HZ_ERR_UNSUPPORTED = 7,  /* combination not provided by this form */
+    HZ_ERR_NO_DEVICE = 8
+};
+
+/* Solution methods ("forms").
+ *  PLAIN_LU  : the reference algorithm on the GPU -- every solve assembles and
+ *              factors each (|kx|,|ky|) operator Dzz_int - sigma I with the
+ *              reference's scaled partial pivoting (LU_Decomposition.cpp:7-72),
+ *              solves particular + homogeneous columns, runs the element
+ *              interface sweep and assembles.  FP64 throughout.
+ *  DIAG      : z-diagonalised form -- Dzz_int = V diag(mu) V^-1 is computed
+ *              once per plan; a solve is two FP64 MFMA contractions per
+ *              column batch plus an O(n) shift per column.  No factorisation.
+ *  MIXED     : FP32 LU per operator (FP64 LU for ill-conditioned low-|k|
+ *              operators) with FP64 residuals and FP64 iterative refinement.
+ */
+enum { HZ_FORM_PLAIN_LU = 0, HZ_FORM_DIAG = 1, HZ_FORM_MIXED = 2 };
+
+typedef struct {
+    int nx, ny;              /* Fourier points in x, y: powers of two >= 2 */
+    int n;                   /* GLL points per element (reference N), >= 3 */
+    int ne;                  /* elements in z (reference size*Ns), >= 1    */
+    double xl, xr, yl, yr;   /* periodic box                                */
+    double zl, zr;           /* z extent; only Lz = zr - zl enters the operator */
+    double lambda;           /* Helmholtz shift (reference Lambda)          */
+    double alpha_l, beta_l, alpha_r, beta_r; /* Robin coefficients          */
+    int form;                /* HZ_FORM_*                                   */
+    int refine_sweeps;       /* MIXED only: refinement sweeps (0 = default 2) */
+    double mixed_fp64_cond;  /* MIXED only: sigma below which an operator keeps
+                                an FP64 factorisation (0 = built-in rule)   */
+} hz_plan_desc;
+
+/* Replaces CreatePDEPlan (NonPeriodic3D_MPI.cpp:40-134): GLL nodes, scaled
+ * Dz/Dzz, the operator list and device workspace.  Must be called with the
+ * target device current. */
+int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan);
+/* Replaces DestroyPDEPlan (NonPeriodic3D_MPI.cpp:139-170). */
+int hz_plan_destroy(hz_plan *plan);
+
+/* Plan geometry.  z (host, nz_total entries) follows the reference exactly:
+ * element e spans [e*Lz/ne, (e+1)*Lz/ne], i.e. z starts at 0, not zl
+ * (NonPeriodic3D_MPI.cpp:111-120). Dz, Dzz (host, n*n, row-major) are the
+ * element matrices scaled by 2ne/Lz and 4ne^2/Lz^2 (:122-131). */
+int hz_plan_nz_total(const hz_plan *plan, int *nz_total);
+int hz_plan_z_nodes(const hz_plan *plan, double *z);
+int hz_plan_operators(const hz_plan *plan, double *Dz, double *Dzz);
+int hz_plan_num_modes(const hz_plan *plan, int *num_modes); /* (nx/2+1)(ny/2+1) */
+/* Changes lambda (and with it every operator) for later solves. */
+int hz_plan_set_lambda(hz_plan *plan, double lambda);
+
+/* Replaces SolvePDE (NonPeriodic3D_MPI.cpp:237-591) for the standard
+ * operator of the reference test callback (TestNonPeriodic3D_MPI.cpp:71-79):
+ * Dzz_int - (lambda + 4 pi^2 iw^2/Lx^2 + 4 pi^2 jw^2/Ly^2) I.
+ * Inputs are not modified (the reference destroys them). */
+int hz_solve(hz_plan *plan, const double *f, const double *g_l, const double *g_r,
+             double *u, hz_stream_t stream);
+
+/* SolvePDE with caller-assembled operators (the reference's callback
+ * contract, NonPeriodic3D_MPI.cpp:248-253): mode m = iw*(ny/2+1) + jw.
+ *  op_diag != NULL, op_full == NULL : operator m = Dzz_int with its diagonal
+ *     replaced by op_diag[m*(n-2) .. +n-2)  (device, float64)
+ *  op_full != NULL : operator m = op_full[m*(n-2)^2 ..] row-major (device).
+ * PLAIN_LU form only. */
+int hz_solve_operators(hz_plan *plan, const double *f, const double *g_l,
+                       const double *g_r, const double *op_diag,
+                       const double *op_full, double *u, hz_stream_t stream);
+
+/* Status of the last solve's factorisations: after the stream has finished,
+ * returns HZ_ERR_SINGULAR if any operator failed the reference's pivot test
+ * (|scaled pivot| <= 1e-12).  Synchronises the stream. */
+int hz_solve_status(hz_plan *plan, hz_stream_t stream);
+
+/* Adjoint (transpose) of the solve map (f, g_l, g_r) -> u at fixed lambda:
+ * given ubar = dL/du (device, [nx][ny][nz_total]), writes
+ *   fbar  = dL/df      [nx][ny][nz_total]  (zero on element end nodes)
+ *   glbar = dL/dg_l, grbar = dL/dg_r   [nx][ny]
+ * and, if u != NULL (the forward solution) and lambda_bar != NULL (host),
+ *   *lambda_bar = dL/dlambda = sum over nodes of fbar * u.
+ * Single element (ne == 1), DIAG and PLAIN_LU forms. */
+int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
+               double *glbar, double *grbar, double *lambda_bar, hz_stream_t stream);
+
+/* Replaces FirstDerivativeZ / SecondDerivativeZ (NonPeriodic3D_MPI.cpp:632-683):
+ * out = Dz u (order 1) or Dzz u (order 2) element by element; at an element
+ * interface the later element's value is stored. */
+int hz_derivative_z(hz_plan *plan, int order, const double *u, double *out,
+                    hz_stream_t stream);
+
+/* Replaces FourierTransformXY (Fourier_Quadrature.cpp:1040-1155) for a
+ * stack of `lanes` planes: isign = +1 maps real in[nx][ny][lanes] to the half
+ * spectrum re/im[nx][ny/2+1][lanes] (unnormalised, exp(-2 pi i jk/L), x index
+ * in FFT order); isign = -1 maps re/im (overwritten) back to real
+ * out[nx][ny][lanes] with the 1/(nx ny) factor.  Device pointers. */
+int hz_fourier_xy(hz_plan *plan, int isign, const double *in, double *re, double *im,
+                  double *out, int lanes, hz_stream_t stream);
+
+/* Batched LU in the reference's storage: for `count` row-major n x n
+ * matrices A (device), factor in place with LU_Decomposition.cpp:7-72's
+ * pivot rule; piv[count*(n-1)] (device int32) and det[count] (device, may
+ * be NULL) as the reference's npivot and determinant; status[count]
+ * (device int32, may be NULL) = 0 or HZ_ERR_SINGULAR. */
+int hz_lu_factor_batched(double *A, int n, int count, int *piv, double *det,
+                         int *status, hz_stream_t stream);
+/* LU_Decomposition.cpp:78-108 for `count` systems: b[count*n] overwritten. */
+int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int count,
+                        hz_stream_t stream);
+
+/* Many independent problems of uneven size in one launch sequence.
+ * plans[p] are DIAG-form plans; f[p], gl[p], gr[p], u[p] device arrays in
+ * each plan's layout.  The first call records a HIP graph for this exact
+ * set of plans and buffers; later calls with the same arguments replay it. */
+typedef struct hz_batch hz_batch;
+int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch);
+int hz_batch_solve(hz_batch *batch, const double *const *f, const double *const *gl,
+                   const double *const *gr, double *const *u, hz_stream_t stream);
+int hz_batch_destroy(hz_batch *batch);
+
+/* Per-sweep refinement history of the last MIXED solve: max-norm of the
+ * z-residual relative to the right-hand side after each sweep (host). */
+int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
+
+/* Host-only utilities (no device needed): the plan-time setup that
+ * hz_plan_create performs, exposed so it can be checked on any machine.
+ * hz_host_plan_operators: GLL nodes on [-1,1] (x[n]), the z array
+ * (z[ne*(n-1)+1]) and the scaled Dz, Dzz (n*n each); any pointer may be NULL.
+ * hz_host_eigen: Dzz_int = V diag(mu) V^-1 for the (n-2)x(n-2) interior
+ * operator of the same plan (row-major V, Vinv; mu ascending). */
+int hz_host_plan_operators(int n, int ne, double lz, double *x, double *z, double *Dz,
+                           double *Dzz);
+int hz_host_eigen(int n, int ne, double lz, double *mu, double *V, double *Vinv);
+
+/* Stage timing with HIP events on the solve stream (measurement hook).
+ * While enabled, every solve records an event before and after each of its
+ * kernel stages; hz_plan_profile_read synchronises, returns the mean
+ * milliseconds per stage over the recorded solves and clears the record.
+ * Stage names (one per slot) are written to `names` as a ';'-joined list. */
+int hz_plan_profile(hz_plan *plan, int enable);
+int hz_plan_profile_read(hz_plan *plan, double *stage_ms, int max_stages, int *nstages,
+                         int *nsolves, char *names, int names_len);
+
+const char *hz_last_error_string(void);
+const char *hz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HZ_H */

@@ -1,0 +1,194 @@
+// fft.hip -- batched line FFTs over the x or y axis of z-fastest fields.
+//
+// The fields are [a][b][K] with K (z) contiguous, so a transform along a or
+// b is a batch of K independent lines whose same-index points sit in one
+// contiguous run of K doubles.  A work-group takes W consecutive z lanes of
+// one line-batch and the whole transform length L:
+//   stage 1: each thread loads R1 points x[n2 + R2*n1] of its lane with
+//            coalesced W-wide loads, does an R1-point DFT in registers and
+//            multiplies by the twiddle W_L^(n2*k1);
+//   exchange through LDS (an L x W tile, lane-fastest, conflict free);
+//   stage 2: R2-point DFTs in registers, stores X[k1 + R1*k2] coalesced.
+// One HBM read and one HBM write per element; the LDS tile is the only
+// transpose.  Forward transforms are unnormalised with exp(-2 pi i jk/L);
+// inverse ones carry 1/L, as the reference's FourierTransformX/Y
+// (Fourier_Quadrature.cpp:778, :809-817).  R2C keeps points 0..L/2; C2R
+// rebuilds the upper half by conjugate symmetry and keeps the real part.
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+constexpr int bitrev(int i, int bits)
+{
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+// In-register R-point DFT, radix-2 decimation in time.  tw holds the
+// L-point table; W_R^t = tw[t * (L/R)].  SIGN = -1 forward, +1 inverse.
+template <int R, int L, int SIGN>
+__device__ __forceinline__ void dft_regs(double (&re)[R], double (&im)[R], const double2 *__restrict__ tw)
+{
+    constexpr int bits = ilog2(R);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int j = bitrev(i, bits);
+        if (j > i) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int j = 0; j < len / 2; ++j) {
+                const int a = i + j, b = i + j + len / 2;
+                double xr = re[b], xi = im[b];
+                if (j != 0) {
+                    const double2 w = tw[j * (L / len)];
+                    const double wr = w.x, wi = SIGN * w.y;
+                    const double tr = xr * wr - xi * wi;
+                    xi = xr * wi + xi * wr;
+                    xr = tr;
+                }
+                re[b] = re[a] - xr;
+                im[b] = im[a] - xi;
+                re[a] = re[a] + xr;
+                im[a] = im[a] + xi;
+            }
+        }
+    }
+}
+
+template <int L, int R1, int R2, int W, int KIND>
+__global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArgs a)
+{
+    static_assert(R1 * R2 == L, "radix split");
+    constexpr int SIGN = (KIND == FFT_R2C || KIND == FFT_C2C_FWD) ? -1 : 1;
+    __shared__ double tre[L * W], tim[L * W];
+
+    const int w = threadIdx.x % W, q = threadIdx.x / W;
+    const int lane = blockIdx.x * W + w;
+    const int batch = blockIdx.y;
+    const bool live = lane < a.lanes;
+    const double2 *__restrict__ tw = a.tw;
+
+    if (q < R2) {
+        const int n2 = q;
+        double re[R1], im[R1];
+        const long long base = (long long)batch * a.in_batch_stride + lane;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+            const int p = n2 + R2 * n1;
+            double vr = 0.0, vi = 0.0;
+            if (live) {
+                if (KIND == FFT_R2C) {
+                    vr = a.in_re[base + (long long)p * a.in_point_stride];
+                } else if (KIND == FFT_C2R) {
+                    const int pp = p <= L / 2 ? p : L - p;
+                    const long long off = base + (long long)pp * a.in_point_stride;
+                    vr = a.in_re[off];
+                    vi = a.in_im[off];
+                    if (p > L / 2) vi = -vi;
+                } else {
+                    const long long off = base + (long long)p * a.in_point_stride;
+                    vr = a.in_re[off];
+                    vi = a.in_im[off];
+                }
+            }
+            re[n1] = vr;
+            im[n1] = vi;
+        }
+        dft_regs<R1, L, SIGN>(re, im, tw);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            double vr = re[k1], vi = im[k1];
+            const int t = (n2 * k1) % L;
+            if (t != 0) {
+                const double2 tt = tw[t];
+                const double wr = tt.x, wi = SIGN * tt.y;
+                const double xr = vr * wr - vi * wi;
+                vi = vr * wi + vi * wr;
+                vr = xr;
+            }
+            tre[(k1 * R2 + n2) * W + w] = vr;
+            tim[(k1 * R2 + n2) * W + w] = vi;
+        }
+    }
+    __syncthreads();
+    if (q < R1) {
+        const int k1 = q;
+        double re[R2], im[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            re[n2] = tre[(k1 * R2 + n2) * W + w];
+            im[n2] = tim[(k1 * R2 + n2) * W + w];
+        }
+        dft_regs<R2, L, SIGN>(re, im, tw);
+        if (live) {
+            const long long base = (long long)batch * a.out_batch_stride + lane;
+            constexpr double scale = 1.0 / L;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const int kk = k1 + R1 * k2;
+                const long long off = base + (long long)kk * a.out_point_stride;
+                if (KIND == FFT_R2C) {
+                    if (kk <= L / 2) {
+                        a.out_re[off] = re[k2];
+                        a.out_im[off] = im[k2];
+                    }
+                } else if (KIND == FFT_C2R) {
+                    a.out_re[off] = re[k2] * scale;
+                } else if (KIND == FFT_C2C_INV) {
+                    a.out_re[off] = re[k2] * scale;
+                    a.out_im[off] = im[k2] * scale;
+                } else {
+                    a.out_re[off] = re[k2];
+                    a.out_im[off] = im[k2];
+                }
+            }
+        }
+    }
+}
+
+template <int L, int R1, int R2, int W>
+int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
+{
+    constexpr int Q = R1 > R2 ? R1 : R2;
+    dim3 grid((a.lanes + W - 1) / W, a.nbatch), block(W * Q);
+    switch (kind) {
+    case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
+    case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
+    case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a); break;
+    case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a); break;
+    }
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+}  // namespace
+
+int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
+{
+    if (a.nbatch <= 0 || a.lanes <= 0) return HZ_OK;
+    switch (L) {
+    case 2: return launch_kind<2, 2, 1, 16>(kind, a, s);
+    case 4: return launch_kind<4, 4, 1, 16>(kind, a, s);
+    case 8: return launch_kind<8, 8, 1, 16>(kind, a, s);
+    case 16: return launch_kind<16, 4, 4, 16>(kind, a, s);
+    case 32: return launch_kind<32, 8, 4, 16>(kind, a, s);
+    case 64: return launch_kind<64, 8, 8, 16>(kind, a, s);
+    case 128: return launch_kind<128, 16, 8, 16>(kind, a, s);
+    case 256: return launch_kind<256, 16, 16, 16>(kind, a, s);
+    case 512: return launch_kind<512, 32, 16, 8>(kind, a, s);
+    case 1024: return launch_kind<1024, 32, 32, 4>(kind, a, s);
+    default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [2, 1024]");
+    }
+}
+
+}  // namespace hz

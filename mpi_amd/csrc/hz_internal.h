@@ -1,0 +1,102 @@
+// hz_internal.h -- plan object and internal interfaces of libhz.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hz.h"
+
+namespace hz {
+
+// ---- host-side plan setup (gll.cpp, eig.cpp) ------------------------------
+double legendre(double x, int n, int *status);
+int gll_nodes(int N, std::vector<double> &x);
+int gll_weights(const std::vector<double> &x, std::vector<double> &w);
+int gll_derivative_matrices(const std::vector<double> &x, std::vector<double> &D,
+                            std::vector<double> &D2);
+int plan_operators(int N, int ne, double Lz, std::vector<double> &z, std::vector<double> &Dz,
+                   std::vector<double> &Dzz);
+double mode_shift(double lambda, int iw, int jw, double Lx, double Ly);
+int eigen_decompose(const std::vector<double> &A, int n, std::vector<double> &mu,
+                    std::vector<double> &V, std::vector<double> &Vinv);
+
+// ---- error reporting --------------------------------------------------------
+int set_error(int code, const std::string &msg);
+#define HZ_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::hz::set_error(HZ_ERR_HIP, std::string(#call) + ": " +           \
+                                                   hipGetErrorString(e_));            \
+    } while (0)
+
+// ---- spectral data layout ----------------------------------------------------
+// Physical fields: [nx][ny][K] (K = nz_total, z fastest).
+// Half spectrum (R2C along y then C2C along x), split real/imaginary planes:
+//   re[(i*M + m)*K + k], im[...]     i < nx (FFT order), m < M = ny/2+1
+// Boundary spectra: [side][nx][M] split planes, side 0 = g_l, 1 = g_r.
+struct Spectrum {
+    double *re = nullptr, *im = nullptr;
+};
+
+// Arguments of the FFT line kernels (fft.hip).
+enum FftKind { FFT_R2C = 0, FFT_C2R = 1, FFT_C2C_FWD = 2, FFT_C2C_INV = 3 };
+struct FftArgs {
+    const double *in_re, *in_im;   // in_im unused for R2C
+    double *out_re, *out_im;       // out_im unused for C2R
+    long long in_batch_stride, in_point_stride;
+    long long out_batch_stride, out_point_stride;
+    int lanes;                     // contiguous lanes (K)
+    int nbatch;
+    const double2 *tw;             // tw[t] = (cos 2 pi t/L, sin 2 pi t/L)
+};
+int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s);
+
+// Arguments of the plain-LU mode kernel (lu.hip).
+struct LuModeArgs {
+    int nx, ny, M, K, N, n, ne;
+    const double *DzzT;      // interior operator transposed: DzzT[j*n + i] = Dzz[i+1][j+1]
+    const double *Dzz;       // full N x N
+    const double *Dz;        // full N x N
+    const double *shift;     // [modes] or null
+    const double *op_diag;   // [modes][n] or null
+    const double *op_full;   // [modes][n][n] or null
+    double *spec_re, *spec_im;
+    const double *bc_re, *bc_im;
+    double alpha_l, beta_l, alpha_r, beta_r;
+    int *status;             // [modes]
+};
+int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s);
+
+struct LuFactorArgs {
+    double *A;
+    int n, count;
+    int *piv;
+    double *det;
+    int *status;
+};
+int launch_lu_factor(const LuFactorArgs &a, hipStream_t s);
+int launch_lu_solve(const double *LU, const int *piv, double *b, int n, int count, hipStream_t s);
+
+}  // namespace hz
+
+// ---- the plan ----------------------------------------------------------------
+struct hz_plan {
+    hz_plan_desc d{};
+    int device = 0;
+    int nx = 0, ny = 0, N = 0, n = 0, ne = 0, K = 0, M = 0, modes = 0;
+    double Lx = 0, Ly = 0, Lz = 0;
+    std::vector<double> z, Dz, Dzz, shift;
+    // device
+    double *d_Dz = nullptr, *d_Dzz = nullptr, *d_DzzT = nullptr, *d_shift = nullptr;
+    double2 *d_twx = nullptr, *d_twy = nullptr;
+    hz::Spectrum spec, bc;
+    int *d_status = nullptr;
+    // stage profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;      // events in use, per solve: nst+1
+    std::vector<std::vector<hipEvent_t>> ev_solves;
+    std::vector<std::string> stage_names;
+};

@@ -1,0 +1,467 @@
+// plan.cpp -- the C ABI (include/hz.h): plan construction, device workspace
+// and the solve pipelines.
+//
+// Solve pipeline (all forms), z-fastest fields, half spectrum in x-y:
+//   f  --R2C along y-->  spec  --C2C along x-->  spec         (fft.hip)
+//   g_l, g_r: the same two passes into the boundary spectra
+//   spec --mode solve (form-specific)--> spec                  (lu.hip / diag.hip / mixed.hip)
+//   spec --C2C^-1 along x--> spec --C2R along y--> u
+#include "hz_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+namespace hz {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace hz
+
+using namespace hz;
+
+namespace {
+
+bool is_pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+
+template <class T>
+int dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return HZ_OK;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return set_error(HZ_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return HZ_OK;
+}
+
+template <class T>
+int upload(T **p, const std::vector<T> &v)
+{
+    int st = dmalloc(p, v.size());
+    if (st) return st;
+    HZ_HIP(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HZ_OK;
+}
+
+std::vector<double2> twiddles(int L)
+{
+    std::vector<double2> t(L);
+    for (int i = 0; i < L; ++i) {
+        const long double ang = 2.0L * 3.14159265358979323846264338327950288L * i / L;
+        t[i].x = (double)cosl(ang);
+        t[i].y = (double)sinl(ang);
+    }
+    return t;
+}
+
+void free_plan(hz_plan *p)
+{
+    if (!p) return;
+    for (auto &v : p->ev_solves)
+        for (auto e : v) (void)hipEventDestroy(e);
+    (void)hipFree(p->d_Dz);
+    (void)hipFree(p->d_Dzz);
+    (void)hipFree(p->d_DzzT);
+    (void)hipFree(p->d_shift);
+    (void)hipFree(p->d_twx);
+    (void)hipFree(p->d_twy);
+    (void)hipFree(p->spec.re);
+    (void)hipFree(p->spec.im);
+    (void)hipFree(p->bc.re);
+    (void)hipFree(p->bc.im);
+    (void)hipFree(p->d_status);
+    delete p;
+}
+
+int compute_shifts(hz_plan *p)
+{
+    p->shift.resize(p->modes);
+    for (int iw = 0; iw <= p->nx / 2; ++iw)
+        for (int jw = 0; jw < p->M; ++jw)
+            p->shift[(size_t)iw * p->M + jw] = mode_shift(p->d.lambda, iw, jw, p->Lx, p->Ly);
+    HZ_HIP(hipMemcpy(p->d_shift, p->shift.data(), sizeof(double) * p->modes, hipMemcpyHostToDevice));
+    return HZ_OK;
+}
+
+// Forward x-y transform of a [nx][ny][lanes] field into split half spectra.
+int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s)
+{
+    FftArgs a{};
+    a.in_re = in;
+    a.out_re = re;
+    a.out_im = im;
+    a.in_batch_stride = (long long)p->ny * lanes;
+    a.in_point_stride = lanes;
+    a.out_batch_stride = (long long)p->M * lanes;
+    a.out_point_stride = lanes;
+    a.lanes = lanes;
+    a.nbatch = p->nx;
+    a.tw = p->d_twy;
+    int st = launch_fft(p->ny, FFT_R2C, a, s);
+    if (st) return st;
+    FftArgs b{};
+    b.in_re = re;
+    b.in_im = im;
+    b.out_re = re;
+    b.out_im = im;
+    b.in_batch_stride = lanes;
+    b.in_point_stride = (long long)p->M * lanes;
+    b.out_batch_stride = lanes;
+    b.out_point_stride = (long long)p->M * lanes;
+    b.lanes = lanes;
+    b.nbatch = p->M;
+    b.tw = p->d_twx;
+    return launch_fft(p->nx, FFT_C2C_FWD, b, s);
+}
+
+int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipStream_t s)
+{
+    FftArgs b{};
+    b.in_re = re;
+    b.in_im = im;
+    b.out_re = re;
+    b.out_im = im;
+    b.in_batch_stride = lanes;
+    b.in_point_stride = (long long)p->M * lanes;
+    b.out_batch_stride = lanes;
+    b.out_point_stride = (long long)p->M * lanes;
+    b.lanes = lanes;
+    b.nbatch = p->M;
+    b.tw = p->d_twx;
+    int st = launch_fft(p->nx, FFT_C2C_INV, b, s);
+    if (st) return st;
+    FftArgs a{};
+    a.in_re = re;
+    a.in_im = im;
+    a.out_re = out;
+    a.in_batch_stride = (long long)p->M * lanes;
+    a.in_point_stride = lanes;
+    a.out_batch_stride = (long long)p->ny * lanes;
+    a.out_point_stride = lanes;
+    a.lanes = lanes;
+    a.nbatch = p->nx;
+    a.tw = p->d_twy;
+    return launch_fft(p->ny, FFT_C2R, a, s);
+}
+
+int forward_bc(hz_plan *p, const double *gl, const double *gr, hipStream_t s)
+{
+    const size_t side = (size_t)p->nx * p->M;
+    int st = forward_xy(p, gl, p->bc.re, p->bc.im, 1, s);
+    if (st) return st;
+    return forward_xy(p, gr, p->bc.re + side, p->bc.im + side, 1, s);
+}
+
+// Records stage boundaries of one solve when profiling is on.
+struct StageRecorder {
+    hz_plan *p;
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::string> names;
+    StageRecorder(hz_plan *plan, hipStream_t stream) : p(plan), s(stream)
+    {
+        if (p->profiling) mark("");
+    }
+    void mark(const char *stage_done)
+    {
+        if (!p->profiling) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        if (stage_done[0]) names.push_back(stage_done);
+    }
+    ~StageRecorder()
+    {
+        if (!p->profiling || ev.size() < 2) {
+            for (auto e : ev) (void)hipEventDestroy(e);
+            return;
+        }
+        if (p->stage_names.empty()) p->stage_names = names;
+        if (names == p->stage_names && p->ev_solves.size() < 4096) {
+            p->ev_solves.push_back(ev);
+        } else {
+            for (auto e : ev) (void)hipEventDestroy(e);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *hz_last_error_string(void) { return g_last_error.c_str(); }
+const char *hz_version(void) { return "hz 0.1 (gfx950)"; }
+
+int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
+{
+    if (!desc || !plan) return set_error(HZ_ERR_ARG, "null argument");
+    *plan = nullptr;
+    const hz_plan_desc &d = *desc;
+    if (!is_pow2(d.nx) || !is_pow2(d.ny))
+        return set_error(HZ_ERR_NOT_POW2, "N has to be 2^n for some integer n.");
+    if (d.nx > 1024 || d.ny > 1024) return set_error(HZ_ERR_UNSUPPORTED, "nx, ny must be <= 1024");
+    if (d.n < 3) return set_error(HZ_ERR_N_TOO_SMALL, "Number of points has to be more than or equal to 3.");
+    if (d.ne < 1) return set_error(HZ_ERR_ARG, "ne must be >= 1");
+    if (!(d.xr > d.xl) || !(d.yr > d.yl) || !(d.zr > d.zl)) return set_error(HZ_ERR_ARG, "empty domain");
+    if (d.form < HZ_FORM_PLAIN_LU || d.form > HZ_FORM_MIXED) return set_error(HZ_ERR_ARG, "unknown form");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(HZ_ERR_NO_DEVICE, "no HIP device");
+
+    hz_plan *p = new hz_plan();
+    p->d = d;
+    HZ_HIP(hipGetDevice(&p->device));
+    p->nx = d.nx;
+    p->ny = d.ny;
+    p->N = d.n;
+    p->n = d.n - 2;
+    p->ne = d.ne;
+    p->K = d.ne * (d.n - 1) + 1;
+    p->M = d.ny / 2 + 1;
+    p->modes = (d.nx / 2 + 1) * p->M;
+    p->Lx = d.xr - d.xl;
+    p->Ly = d.yr - d.yl;
+    p->Lz = d.zr - d.zl;
+    int st = plan_operators(p->N, p->ne, p->Lz, p->z, p->Dz, p->Dzz);
+    if (st) {
+        free_plan(p);
+        return set_error(st, "GLL setup failed");
+    }
+    std::vector<double> DzzT((size_t)p->n * p->n);
+    for (int i = 0; i < p->n; ++i)
+        for (int j = 0; j < p->n; ++j) DzzT[(size_t)j * p->n + i] = p->Dzz[(size_t)(i + 1) * p->N + (j + 1)];
+    const size_t specN = (size_t)p->nx * p->M * p->K, bcN = (size_t)2 * p->nx * p->M;
+    if ((st = upload(&p->d_Dz, p->Dz)) || (st = upload(&p->d_Dzz, p->Dzz)) ||
+        (st = upload(&p->d_DzzT, DzzT)) || (st = dmalloc(&p->d_shift, p->modes)) ||
+        (st = upload(&p->d_twx, twiddles(p->nx))) || (st = upload(&p->d_twy, twiddles(p->ny))) ||
+        (st = dmalloc(&p->spec.re, specN)) || (st = dmalloc(&p->spec.im, specN)) ||
+        (st = dmalloc(&p->bc.re, bcN)) || (st = dmalloc(&p->bc.im, bcN)) ||
+        (st = dmalloc(&p->d_status, p->modes)) || (st = compute_shifts(p))) {
+        free_plan(p);
+        return st;
+    }
+    HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
+    *plan = p;
+    return HZ_OK;
+}
+
+int hz_plan_destroy(hz_plan *plan)
+{
+    free_plan(plan);
+    return HZ_OK;
+}
+
+int hz_plan_nz_total(const hz_plan *p, int *nz)
+{
+    if (!p || !nz) return set_error(HZ_ERR_ARG, "null argument");
+    *nz = p->K;
+    return HZ_OK;
+}
+
+int hz_plan_z_nodes(const hz_plan *p, double *z)
+{
+    if (!p || !z) return set_error(HZ_ERR_ARG, "null argument");
+    std::memcpy(z, p->z.data(), sizeof(double) * p->z.size());
+    return HZ_OK;
+}
+
+int hz_plan_operators(const hz_plan *p, double *Dz, double *Dzz)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    if (Dz) std::memcpy(Dz, p->Dz.data(), sizeof(double) * p->Dz.size());
+    if (Dzz) std::memcpy(Dzz, p->Dzz.data(), sizeof(double) * p->Dzz.size());
+    return HZ_OK;
+}
+
+int hz_plan_num_modes(const hz_plan *p, int *m)
+{
+    if (!p || !m) return set_error(HZ_ERR_ARG, "null argument");
+    *m = p->modes;
+    return HZ_OK;
+}
+
+int hz_plan_set_lambda(hz_plan *p, double lambda)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    p->d.lambda = lambda;
+    return compute_shifts(p);
+}
+
+static int solve_impl(hz_plan *p, const double *f, const double *gl, const double *gr,
+                      const double *op_diag, const double *op_full, double *u, hipStream_t s)
+{
+    if (!p || !f || !gl || !gr || !u) return set_error(HZ_ERR_ARG, "null argument");
+    StageRecorder rec(p, s);
+    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s);
+    rec.mark("fft_forward");
+    if (!st) st = forward_bc(p, gl, gr, s);
+    rec.mark("fft_faces");
+    if (st) return st;
+    if (p->d.form == HZ_FORM_PLAIN_LU || op_diag || op_full) {
+        LuModeArgs a{};
+        a.nx = p->nx;
+        a.ny = p->ny;
+        a.M = p->M;
+        a.K = p->K;
+        a.N = p->N;
+        a.n = p->n;
+        a.ne = p->ne;
+        a.DzzT = p->d_DzzT;
+        a.Dzz = p->d_Dzz;
+        a.Dz = p->d_Dz;
+        a.shift = p->d_shift;
+        a.op_diag = op_diag;
+        a.op_full = op_full;
+        a.spec_re = p->spec.re;
+        a.spec_im = p->spec.im;
+        a.bc_re = p->bc.re;
+        a.bc_im = p->bc.im;
+        a.alpha_l = p->d.alpha_l;
+        a.beta_l = p->d.beta_l;
+        a.alpha_r = p->d.alpha_r;
+        a.beta_r = p->d.beta_r;
+        a.status = p->d_status;
+        st = launch_lu_modes(a, p->modes, s);
+        rec.mark("lu_modes");
+    } else {
+        st = set_error(HZ_ERR_UNSUPPORTED, "form not built yet");
+    }
+    if (st) return st;
+    st = inverse_xy(p, p->spec.re, p->spec.im, u, p->K, s);
+    rec.mark("fft_inverse");
+    return st;
+}
+
+int hz_plan_profile(hz_plan *p, int enable)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    p->profiling = enable != 0;
+    return HZ_OK;
+}
+
+int hz_plan_profile_read(hz_plan *p, double *stage_ms, int max_stages, int *nstages, int *nsolves,
+                         char *names, int names_len)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    const int ns = (int)p->stage_names.size();
+    std::vector<double> acc(ns, 0.0);
+    for (auto &v : p->ev_solves) {
+        for (int i = 0; i < ns && i + 1 < (int)v.size(); ++i) {
+            HZ_HIP(hipEventSynchronize(v[i + 1]));
+            float ms = 0;
+            HZ_HIP(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+            acc[i] += ms;
+        }
+    }
+    const int count = (int)p->ev_solves.size();
+    for (int i = 0; i < ns && i < max_stages; ++i) stage_ms[i] = count ? acc[i] / count : 0.0;
+    if (nstages) *nstages = ns;
+    if (nsolves) *nsolves = count;
+    if (names && names_len > 0) {
+        std::string j;
+        for (int i = 0; i < ns; ++i) j += (i ? ";" : "") + p->stage_names[i];
+        std::snprintf(names, names_len, "%s", j.c_str());
+    }
+    for (auto &v : p->ev_solves)
+        for (auto e : v) (void)hipEventDestroy(e);
+    p->ev_solves.clear();
+    p->stage_names.clear();
+    return HZ_OK;
+}
+
+int hz_solve(hz_plan *p, const double *f, const double *gl, const double *gr, double *u,
+             hz_stream_t stream)
+{
+    return solve_impl(p, f, gl, gr, nullptr, nullptr, u, (hipStream_t)stream);
+}
+
+int hz_solve_operators(hz_plan *p, const double *f, const double *gl, const double *gr,
+                       const double *op_diag, const double *op_full, double *u, hz_stream_t stream)
+{
+    if (!op_diag && !op_full) return set_error(HZ_ERR_ARG, "no operators given");
+    return solve_impl(p, f, gl, gr, op_diag, op_full, u, (hipStream_t)stream);
+}
+
+int hz_solve_status(hz_plan *p, hz_stream_t stream)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    HZ_HIP(hipStreamSynchronize((hipStream_t)stream));
+    std::vector<int> st(p->modes);
+    HZ_HIP(hipMemcpy(st.data(), p->d_status, sizeof(int) * p->modes, hipMemcpyDeviceToHost));
+    for (int m = 0; m < p->modes; ++m)
+        if (st[m]) return set_error(HZ_ERR_SINGULAR, "ERROR: Matrix is singular!");
+    return HZ_OK;
+}
+
+int hz_host_plan_operators(int n, int ne, double lz, double *x, double *z, double *Dz, double *Dzz)
+{
+    if (n < 2) return set_error(HZ_ERR_N_TOO_SMALL, "Number of points has to be more than or equal to 2.");
+    if (ne < 1 || !(lz > 0)) return set_error(HZ_ERR_ARG, "bad element count or length");
+    std::vector<double> xi, zz, a, b;
+    int st = gll_nodes(n, xi);
+    if (st) return set_error(st, "GLL setup failed");
+    st = plan_operators(n, ne, lz, zz, a, b);
+    if (st) return set_error(st, "GLL setup failed");
+    if (x) std::memcpy(x, xi.data(), sizeof(double) * xi.size());
+    if (z) std::memcpy(z, zz.data(), sizeof(double) * zz.size());
+    if (Dz) std::memcpy(Dz, a.data(), sizeof(double) * a.size());
+    if (Dzz) std::memcpy(Dzz, b.data(), sizeof(double) * b.size());
+    return HZ_OK;
+}
+
+int hz_host_eigen(int n, int ne, double lz, double *mu, double *V, double *Vinv)
+{
+    if (n < 3) return set_error(HZ_ERR_N_TOO_SMALL, "need n >= 3");
+    std::vector<double> zz, a, b;
+    int st = plan_operators(n, ne, lz, zz, a, b);
+    if (st) return set_error(st, "GLL setup failed");
+    const int m = n - 2;
+    std::vector<double> A((size_t)m * m), ev, vv, vi;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = b[(size_t)(i + 1) * n + (j + 1)];
+    st = eigen_decompose(A, m, ev, vv, vi);
+    if (st) return set_error(st, "eigendecomposition did not converge");
+    if (mu) std::memcpy(mu, ev.data(), sizeof(double) * m);
+    if (V) std::memcpy(V, vv.data(), sizeof(double) * vv.size());
+    if (Vinv) std::memcpy(Vinv, vi.data(), sizeof(double) * vi.size());
+    return HZ_OK;
+}
+
+int hz_fourier_xy(hz_plan *p, int isign, const double *in, double *re, double *im, double *out,
+                  int lanes, hz_stream_t stream)
+{
+    if (!p || !re || !im || lanes < 1) return set_error(HZ_ERR_ARG, "bad argument");
+    if (isign == 1) {
+        if (!in) return set_error(HZ_ERR_ARG, "null input");
+        return forward_xy(p, in, re, im, lanes, (hipStream_t)stream);
+    }
+    if (isign == -1) {
+        if (!out) return set_error(HZ_ERR_ARG, "null output");
+        return inverse_xy(p, re, im, out, lanes, (hipStream_t)stream);
+    }
+    return set_error(HZ_ERR_ARG, "isign must be +1 or -1");
+}
+
+int hz_lu_factor_batched(double *A, int n, int count, int *piv, double *det, int *status,
+                         hz_stream_t stream)
+{
+    if (!A || n < 1 || count < 0) return set_error(HZ_ERR_ARG, "bad argument");
+    LuFactorArgs a{A, n, count, piv, det, status};
+    return launch_lu_factor(a, (hipStream_t)stream);
+}
+
+int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int count,
+                        hz_stream_t stream)
+{
+    if (!LU || !piv || !b || n < 1) return set_error(HZ_ERR_ARG, "bad argument");
+    return launch_lu_solve(LU, piv, b, n, count, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+"""Host-side plan object over the C ABI (include/hz.h).
+
+``HelmholtzPlan`` owns one ``hz_plan``.  Fields are torch float64 tensors on
+the current HIP device, shaped ``[nx, ny, nz_total]`` (z fastest) for volume
+data and ``[nx, ny]`` for face data.  Nothing here computes: every call goes
+through libhz.so, and a missing library or device raises.
+
+Reference counterpart: CreatePDEPlan / SolvePDE / DestroyPDEPlan,
+HelmholtzSolvers/DirectPseudoSpectral/src/NonPeriodic3D_MPI.cpp:40-591.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, load
+
+FORMS = {"plain": _lib.FORM_PLAIN_LU, "diag": _lib.FORM_DIAG, "mixed": _lib.FORM_MIXED}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class HelmholtzPlan:
+    def __init__(self, nx, ny, n, ne=1, xl=0.0, xr=2 * math.pi, yl=0.0, yr=2 * math.pi,
+                 zl=-1.0, zr=1.0, lam=1.0, alpha_l=0.0, beta_l=1.0, alpha_r=0.0, beta_r=1.0,
+                 form="plain", refine_sweeps=0, mixed_fp64_cond=0.0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HelmholtzPlan needs a HIP device")
+        self.lib = load()
+        self.desc = _lib.PlanDesc(nx, ny, n, ne, xl, xr, yl, yr, zl, zr, lam, alpha_l, beta_l,
+                                  alpha_r, beta_r, FORMS[form] if isinstance(form, str) else form,
+                                  refine_sweeps, mixed_fp64_cond)
+        self.handle = ctypes.c_void_p()
+        check(self.lib.hz_plan_create(ctypes.byref(self.desc), ctypes.byref(self.handle)))
+        k = ctypes.c_int()
+        check(self.lib.hz_plan_nz_total(self.handle, ctypes.byref(k)))
+        self.nx, self.ny, self.n, self.ne, self.nz = nx, ny, n, ne, k.value
+        m = ctypes.c_int()
+        check(self.lib.hz_plan_num_modes(self.handle, ctypes.byref(m)))
+        self.num_modes = m.value
+        self.form = form
+
+    # ---- geometry ------------------------------------------------------
+    def z_nodes(self):
+        """The reference's z array (starts at 0, NonPeriodic3D_MPI.cpp:111-120)."""
+        z = np.zeros(self.nz)
+        check(self.lib.hz_plan_z_nodes(self.handle, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return z
+
+    def operators(self):
+        Dz = np.zeros((self.n, self.n))
+        Dzz = np.zeros((self.n, self.n))
+        P = ctypes.POINTER(ctypes.c_double)
+        check(self.lib.hz_plan_operators(self.handle, Dz.ctypes.data_as(P), Dzz.ctypes.data_as(P)))
+        return Dz, Dzz
+
+    def grid(self):
+        """Physical node coordinates x[i], y[j], zeta[k] (zeta = z + zl)."""
+        d = self.desc
+        x = d.xl + np.arange(self.nx) * ((d.xr - d.xl) / self.nx)
+        y = d.yl + np.arange(self.ny) * ((d.yr - d.yl) / self.ny)
+        return x, y, self.z_nodes() + d.zl
+
+    def set_lambda(self, lam):
+        check(self.lib.hz_plan_set_lambda(self.handle, float(lam)))
+        self.desc.lam = float(lam)
+
+    # ---- solves --------------------------------------------------------
+    def empty_field(self):
+        return torch.empty((self.nx, self.ny, self.nz), dtype=torch.float64, device="cuda")
+
+    def _check_field(self, t, shape):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+            raise ValueError(f"expected contiguous float64 HIP tensor of shape {shape}")
+
+    def solve(self, f, gl, gr, out=None, stream=None):
+        self._check_field(f, (self.nx, self.ny, self.nz))
+        self._check_field(gl, (self.nx, self.ny))
+        self._check_field(gr, (self.nx, self.ny))
+        u = out if out is not None else self.empty_field()
+        check(self.lib.hz_solve(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(u), _stream_ptr(stream)))
+        return u
+
+    def solve_operators(self, f, gl, gr, op_diag=None, op_full=None, out=None, stream=None):
+        u = out if out is not None else self.empty_field()
+        check(self.lib.hz_solve_operators(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(op_diag),
+                                          _ptr(op_full), _ptr(u), _stream_ptr(stream)))
+        return u
+
+    def status(self, stream=None):
+        check(self.lib.hz_solve_status(self.handle, _stream_ptr(stream)))
+
+    def adjoint(self, ubar, u=None, stream=None):
+        fbar = self.empty_field()
+        glbar = torch.empty((self.nx, self.ny), dtype=torch.float64, device="cuda")
+        grbar = torch.empty_like(glbar)
+        lam_bar = ctypes.c_double(0.0)
+        check(self.lib.hz_adjoint(self.handle, _ptr(ubar), _ptr(u), _ptr(fbar), _ptr(glbar), _ptr(grbar),
+                                  ctypes.byref(lam_bar) if u is not None else None, _stream_ptr(stream)))
+        return fbar, glbar, grbar, (lam_bar.value if u is not None else None)
+
+    def derivative_z(self, u, order=1, out=None, stream=None):
+        o = out if out is not None else self.empty_field()
+        check(self.lib.hz_derivative_z(self.handle, int(order), _ptr(u), _ptr(o), _stream_ptr(stream)))
+        return o
+
+    def fourier_xy(self, x, lanes=None, stream=None):
+        """Forward half-spectrum transform of real x[nx, ny, lanes] -> (re, im)[nx, ny//2+1, lanes]."""
+        lanes = x.shape[2] if x.dim() == 3 else 1
+        M = self.ny // 2 + 1
+        re = torch.empty((self.nx, M, lanes), dtype=torch.float64, device="cuda")
+        im = torch.empty_like(re)
+        check(self.lib.hz_fourier_xy(self.handle, 1, _ptr(x), _ptr(re), _ptr(im), None, lanes,
+                                     _stream_ptr(stream)))
+        return re, im
+
+    def inverse_fourier_xy(self, re, im, stream=None):
+        lanes = re.shape[2]
+        out = torch.empty((self.nx, self.ny, lanes), dtype=torch.float64, device="cuda")
+        re = re.clone()
+        im = im.clone()
+        check(self.lib.hz_fourier_xy(self.handle, -1, None, _ptr(re), _ptr(im), _ptr(out), lanes,
+                                     _stream_ptr(stream)))
+        return out
+
+    def profile(self, enable=True):
+        check(self.lib.hz_plan_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        """Mean milliseconds per kernel stage over the solves recorded since
+        profiling was enabled (HIP events on the solve stream)."""
+        ms = np.zeros(32)
+        ns = ctypes.c_int(0)
+        nsol = ctypes.c_int(0)
+        buf = ctypes.create_string_buffer(1024)
+        check(self.lib.hz_plan_profile_read(self.handle, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 32,
+                                            ctypes.byref(ns), ctypes.byref(nsol), buf, 1024))
+        names = buf.value.decode().split(";") if ns.value else []
+        return dict(zip(names, ms[: ns.value].tolist())), nsol.value
+
+    def mixed_history(self, max_sweeps=16):
+        h = np.zeros(max_sweeps)
+        n = ctypes.c_int(0)
+        check(self.lib.hz_mixed_history(self.handle, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        max_sweeps, ctypes.byref(n)))
+        return h[: n.value]
+
+    def close(self):
+        if self.handle:
+            self.lib.hz_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lu_factor_batched(A, stream=None):
+    """Reference LU (LU_Decomposition.cpp:7-72) of A[count, n, n] in place on the GPU.
+    Returns (pivots[count, n-1] int32, det[count], status[count])."""
+    lib = load()
+    count, n, _ = A.shape
+    piv = torch.zeros((count, max(n - 1, 1)), dtype=torch.int32, device=A.device)
+    det = torch.zeros(count, dtype=torch.float64, device=A.device)
+    st = torch.zeros(count, dtype=torch.int32, device=A.device)
+    check(lib.hz_lu_factor_batched(_ptr(A), n, count, _ptr(piv), _ptr(det), _ptr(st), _stream_ptr(stream)))
+    return piv, det, st
+
+
+def lu_solve_batched(LU, piv, b, stream=None):
+    lib = load()
+    count, n, _ = LU.shape
+    check(lib.hz_lu_solve_batched(_ptr(LU), _ptr(piv), _ptr(b), n, count, _stream_ptr(stream)))
+    return b

@@ -1,0 +1,195 @@
+// ref_driver.cpp -- fixture driver linked against gpritam/MPI's
+// DirectPseudoSpectral sources, compiled where they lie under the reference
+// tree by oracle/ref.mk into oracle/_ref/ (never committed, never shipped).
+//
+// TEST INFRASTRUCTURE ONLY: it writes the golden vectors under tests/golden/
+// that pin the oracle (hz_oracle.c).  It contains no reference code; it only
+// calls the reference's public functions (headers under
+// HelmholtzSolvers/DirectPseudoSpectral/include).
+//
+// Modes (all binary files are raw little-endian float64 / int32):
+//   gll N OUT                       x[N] w[N] D[N*N] Dzz[N*N]
+//   lu n COUNT IN OUT               IN: COUNT*n*n doubles; OUT per matrix:
+//                                   LU[n*n] det, then COUNT*n int32 pivots
+//   fft Nx Ny Nz ISIGN IN OUT       FourierTransformXY on A[2Nx][2Ny][Nz]
+//   solve Nx Ny Nz N xl xr yl yr zl zr al bl ar br lambda IN OUT
+//                                   (run under mpiexec -np P) IN: f[Nx][Ny][Ktot]
+//                                   gl[Nx][Ny] gr[Nx][Ny]; OUT.rankR: this
+//                                   rank's [Ns][Nx][Ny][N] block, then z.
+#include "NonPeriodic3D_MPI.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+static std::vector<double> read_doubles(const char *path, size_t count)
+{
+    std::vector<double> v(count);
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp || std::fread(v.data(), sizeof(double), count, fp) != count) {
+        std::fprintf(stderr, "ref_driver: cannot read %zu doubles from %s\n", count, path);
+        std::exit(2);
+    }
+    std::fclose(fp);
+    return v;
+}
+
+static void write_raw(FILE *fp, const void *p, size_t bytes)
+{
+    if (std::fwrite(p, 1, bytes, fp) != bytes) {
+        std::fprintf(stderr, "ref_driver: write failed\n");
+        std::exit(2);
+    }
+}
+
+static int mode_gll(int argc, char **argv)
+{
+    if (argc < 4) return 2;
+    const int n = std::atoi(argv[2]);
+    double *x, *w, **D, **Dzz;
+    Allocate(x, n);
+    Allocate(w, n);
+    Allocate(D, n, n);
+    Allocate(Dzz, n, n);
+    GaussLobattoLegendrePoints(x, n);
+    GaussLobattoLegendreWeights(w, x, n);
+    FirstDerivativeMatrix(D, x, n);
+    SecondDerivativeMatrix(Dzz, D, n);
+    FILE *fp = std::fopen(argv[3], "wb");
+    write_raw(fp, x, sizeof(double) * n);
+    write_raw(fp, w, sizeof(double) * n);
+    for (int i = 0; i < n; ++i) write_raw(fp, D[i], sizeof(double) * n);
+    for (int i = 0; i < n; ++i) write_raw(fp, Dzz[i], sizeof(double) * n);
+    std::fclose(fp);
+    return 0;
+}
+
+static int mode_lu(int argc, char **argv)
+{
+    if (argc < 6) return 2;
+    const int n = std::atoi(argv[2]), count = std::atoi(argv[3]);
+    std::vector<double> in = read_doubles(argv[4], (size_t)count * n * n);
+    FILE *fp = std::fopen(argv[5], "wb");
+    double **A;
+    int *piv;
+    Allocate(A, n, n);
+    Allocate(piv, n);
+    std::vector<int> pivots;
+    for (int c = 0; c < count; ++c) {
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) A[i][j] = in[((size_t)c * n + i) * n + j];
+        double det = 0.0;
+        LUDecomposition(A, piv, det, n);
+        for (int i = 0; i < n; ++i) write_raw(fp, A[i], sizeof(double) * n);
+        write_raw(fp, &det, sizeof(double));
+        for (int i = 0; i < n; ++i) pivots.push_back(i < n - 1 ? piv[i] : -1);
+    }
+    write_raw(fp, pivots.data(), sizeof(int) * pivots.size());
+    std::fclose(fp);
+    return 0;
+}
+
+static int mode_fft(int argc, char **argv)
+{
+    if (argc < 8) return 2;
+    const int nx = std::atoi(argv[2]), ny = std::atoi(argv[3]), nz = std::atoi(argv[4]);
+    const int isign = std::atoi(argv[5]);
+    std::vector<double> in = read_doubles(argv[6], (size_t)4 * nx * ny * nz);
+    CreateFFTPlan(nx > ny ? nx : ny);
+    FILE *fp = std::fopen(argv[7], "wb");
+    if (nz == 1) {
+        double **A;
+        Allocate(A, 2 * nx, 2 * ny);
+        for (int i = 0; i < 2 * nx; ++i)
+            for (int j = 0; j < 2 * ny; ++j) A[i][j] = in[(size_t)i * 2 * ny + j];
+        FourierTransformXY(A, isign, nx, ny);
+        for (int i = 0; i < 2 * nx; ++i) write_raw(fp, A[i], sizeof(double) * 2 * ny);
+    } else {
+        double ***A;
+        Allocate(A, 2 * nx, 2 * ny, nz);
+        for (int i = 0; i < 2 * nx; ++i)
+            for (int j = 0; j < 2 * ny; ++j)
+                for (int k = 0; k < nz; ++k) A[i][j][k] = in[((size_t)i * 2 * ny + j) * nz + k];
+        FourierTransformXY(A, isign, nx, ny, nz);
+        for (int i = 0; i < 2 * nx; ++i)
+            for (int j = 0; j < 2 * ny; ++j) write_raw(fp, A[i][j], sizeof(double) * nz);
+    }
+    std::fclose(fp);
+    return 0;
+}
+
+// The operator callback of the reference's own test program: interior rows
+// and columns of the scaled Dzz minus the Fourier shift, then the
+// reference's LU factorisation (TestNonPeriodic3D_MPI.cpp:71-79 semantics).
+static void construct_operator(const int iw, const int jw)
+{
+    const double shift = Lambda + 4.0 * PI * PI * iw * iw / (Lx * Lx) + 4.0 * PI * PI * jw * jw / (Ly * Ly);
+    for (int i = 0; i < N - 2; ++i)
+        for (int j = 0; j < N - 2; ++j) Operator[i][j] = Dzz[i + 1][j + 1] - (i == j ? shift : 0.0);
+    LUDecomposition(Operator, npivot, determinant, N - 2);
+}
+
+static int mode_solve(int argc, char **argv)
+{
+    if (argc < 19) return 2;
+    MPI_Init(&argc, &argv);
+    MPI_Comm_rank(MPI_COMM_WORLD, &Rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    const int nx = std::atoi(argv[2]), ny = std::atoi(argv[3]), nz = std::atoi(argv[4]), np = std::atoi(argv[5]);
+    double b[10];
+    for (int i = 0; i < 6; ++i) b[i] = std::atof(argv[6 + i]);
+    CreatePDEPlan(nx, ny, nz, np, b[0], b[1], b[2], b[3], b[4], b[5]);
+    CreateFFTPlan(Maximum(Nx, Ny));
+    alphal = std::atof(argv[12]);
+    betal = std::atof(argv[13]);
+    alphar = std::atof(argv[14]);
+    betar = std::atof(argv[15]);
+    Lambda = std::atof(argv[16]);
+    const int ktot = size * Ns * (N - 1) + 1;
+    const size_t nf = (size_t)Nx * Ny * ktot, np2 = (size_t)Nx * Ny;
+    std::vector<double> in = read_doubles(argv[17], nf + 2 * np2);
+    FieldVariable *V = new FieldVariable[Ns];
+    for (int kk = 0; kk < Ns; ++kk) Allocate(V[kk].phi, 2 * Nx, NY, N);
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k)
+                    rhsPDE[kk][i][j][k] = in[((size_t)i * Ny + j) * ktot + (size_t)(Rank * Ns + kk) * (N - 1) + k];
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j) {
+            leftBC[i][j] = in[nf + (size_t)i * Ny + j];
+            rightBC[i][j] = in[nf + np2 + (size_t)i * Ny + j];
+        }
+    MPI_Barrier(MPI_COMM_WORLD);
+    const double t0 = MPI_Wtime();
+    SolvePDE(V, construct_operator);
+    MPI_Barrier(MPI_COMM_WORLD);
+    const double t1 = MPI_Wtime();
+    std::string path = std::string(argv[18]) + ".rank" + std::to_string(Rank);
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j) write_raw(fp, V[kk].phi[i][j], sizeof(double) * N);
+    write_raw(fp, z, sizeof(double) * (Ns * (N - 1) + 1));
+    std::fclose(fp);
+    if (Rank == 0) std::printf("solve_seconds %.6f Ns %d size %d\n", t1 - t0, Ns, size);
+    for (int kk = 0; kk < Ns; ++kk) Deallocate(V[kk].phi, 2 * Nx, NY, N);
+    delete[] V;
+    DestroyFFTPlan(Maximum(Nx, Ny));
+    DestroyPDEPlan();
+    MPI_Finalize();
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    if (mode == "gll") return mode_gll(argc, argv);
+    if (mode == "lu") return mode_lu(argc, argv);
+    if (mode == "fft") return mode_fft(argc, argv);
+    if (mode == "solve") return mode_solve(argc, argv);
+    std::fprintf(stderr, "ref_driver: unknown mode %s\n", mode.c_str());
+    return 2;
+}

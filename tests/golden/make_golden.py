@@ -1,0 +1,195 @@
+"""Writes the golden vectors under tests/golden/ by running gpritam/MPI's own
+DirectPseudoSpectral code, compiled from the reference sources by
+``make -C oracle ref`` into oracle/_ref/ref_driver (this container only; the
+reference and that binary never reach the GPU machine).
+
+Every fixture stores inputs by their generator (seeds and parameters, plus a
+SHA-256 of the generated input bytes) and the reference's outputs either in
+full (small cases, hex floats) or as a SHA-256 of the exact output bytes
+plus sampled values and norms (large cases).
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+from mpi_amd import workloads as wl  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402  (for z grids / sizes only)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from tests_lu_cases import lu_cases  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+TMP = tempfile.mkdtemp(prefix="golden_")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def hexs(a):
+    return [float(v).hex() for v in np.asarray(a, dtype=np.float64).ravel()]
+
+
+def run(args, **kw):
+    env = dict(os.environ, PATH="/opt/conda/bin:" + os.environ["PATH"])
+    return subprocess.run(args, check=True, capture_output=True, text=True, env=env, **kw)
+
+
+def gll_fixture():
+    out = {}
+    for N in list(range(2, 41)) + [65, 129, 257]:
+        path = os.path.join(TMP, "g.bin")
+        run([REF, "gll", str(N), path])
+        a = np.fromfile(path)
+        x, w, D, Dzz = a[:N], a[N:2 * N], a[2 * N:2 * N + N * N], a[2 * N + N * N:]
+        e = {"sha_x": sha(x), "sha_w": sha(w), "sha_D": sha(D), "sha_Dzz": sha(Dzz),
+             "x": hexs(x), "w": hexs(w), "D_row0": hexs(D[:N]), "Dzz_diag": hexs(Dzz.reshape(N, N).diagonal())}
+        out[str(N)] = e
+    return out
+
+
+def lu_fixture():
+    out = {}
+    for name, n, A in lu_cases():
+        inp = os.path.join(TMP, "lu_in.bin")
+        outp = os.path.join(TMP, "lu_out.bin")
+        A.tofile(inp)
+        run([REF, "lu", str(n), "1", inp, outp])
+        raw = np.fromfile(outp, dtype=np.uint8)
+        dd = raw[: (n * n + 1) * 8].view(np.float64)
+        piv = raw[(n * n + 1) * 8:].view(np.int32)[: n - 1]
+        out[name] = {"n": n, "sha_input": sha(A), "sha_factors": sha(dd[: n * n]), "det": float(dd[-1]).hex(),
+                     "pivots": [int(v) for v in piv], "factors_head": hexs(dd[: min(n * n, 64)])}
+    return out
+
+
+def fft_fixture():
+    out = {}
+    rng = wl.SplitMix64(77)
+    for nx, ny, nz in ((8, 4, 3), (32, 32, 5), (16, 64, 1), (4, 8, 2)):
+        A = np.zeros((2 * nx, 2 * ny, nz))
+        A[:nx, :ny, :] = np.array([rng.normal() for _ in range(nx * ny * nz)]).reshape(nx, ny, nz)
+        inp = os.path.join(TMP, "f_in.bin")
+        outp = os.path.join(TMP, "f_out.bin")
+        A.tofile(inp)
+        rec = {"shape": [nx, ny, nz], "sha_input": sha(A)}
+        for isign in (1, -1):
+            run([REF, "fft", str(nx), str(ny), str(nz), str(isign), inp, outp])
+            r = np.fromfile(outp)
+            rec[f"sha_out_{isign}"] = sha(r)
+            rec[f"head_{isign}"] = hexs(r[:32])
+        out[f"{nx}x{ny}x{nz}"] = rec
+    return out
+
+
+def ref_solve(desc, f, gl, gr):
+    K = po.total_z_points(desc)
+    inp = os.path.join(TMP, "s_in.bin")
+    outp = os.path.join(TMP, "s_out.bin")
+    np.concatenate([f.ravel(), gl.ravel(), gr.ravel()]).tofile(inp)
+    args = [MPIEXEC, "-np", str(desc.size), REF, "solve", str(desc.Nx), str(desc.Ny), str(desc.Nz), str(desc.N)]
+    args += [repr(v) for v in (desc.xl, desc.xr, desc.yl, desc.yr, desc.zl, desc.zr,
+                               desc.alphal, desc.betal, desc.alphar, desc.betar, desc.lam)]
+    args += [inp, outp]
+    res = run(args)
+    Ns = (K - 1) // (desc.size * (desc.N - 1))
+    N = desc.N
+    u = np.zeros((desc.Nx, desc.Ny, K))
+    for r in range(desc.size):
+        a = np.fromfile(f"{outp}.rank{r}")
+        blk = a[: Ns * desc.Nx * desc.Ny * N].reshape(Ns, desc.Nx, desc.Ny, N)
+        for kk in range(Ns):
+            g = (r * Ns + kk) * (N - 1)
+            u[:, :, g:g + N] = blk[kk]
+    secs = float(res.stdout.split()[1])
+    return u, secs
+
+
+def sample_idx(shape, count, seed):
+    rng = wl.SplitMix64(seed)
+    return [[rng.randint(0, s - 1) for s in shape] for _ in range(count)]
+
+
+def solve_case(name, desc, f, gl, gr, exact=None, full=False, gen=None):
+    u, secs = ref_solve(desc, f, gl, gr)
+    rec = {"desc": {k: getattr(desc, k) for k, _ in desc._fields_}, "generator": gen,
+           "sha_f": sha(f), "sha_gl": sha(gl), "sha_gr": sha(gr), "sha_u": sha(u),
+           "max_abs_u": float(np.abs(u).max()), "ref_seconds": secs}
+    if exact is not None:
+        rec["err_vs_exact"] = float(np.abs(u - exact).max())
+    idx = sample_idx(u.shape, 4096 if not full else 256, 99)
+    rec["sample_idx"] = idx
+    rec["sample_u"] = hexs([u[i, j, k] for i, j, k in idx])
+    np_path = os.path.join(HERE, f"solve_{name}.json")
+    with open(np_path, "w") as fp:
+        json.dump(rec, fp, indent=0)
+    print(f"{name}: ref {secs:.3f}s err_vs_exact {rec.get('err_vs_exact')}")
+
+
+def shipped_inputs(size):
+    """The reference's own test program, TestNonPeriodic3D_MPI.cpp:26-128."""
+    PI = 3.1415926535897932
+    d = po.make_desc(64, 32, 250, 10, size=size, xl=0, xr=1, yl=0, yr=1, zl=0, zr=1,
+                     alphal=1.0, betal=0.0, alphar=0.0, betar=1.0, lam=1.0)
+    z = po.z_grid(d)
+    x = np.arange(64) / 64.0
+    y = np.arange(32) / 32.0
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    f = -(64 * PI * PI + 36 * PI * PI + 16 * PI * PI + 1.0) * np.sin(8 * PI * X) * np.cos(6 * PI * Y) * np.cos(4 * PI * Z)
+    XX, YY = np.meshgrid(x, y, indexing="ij")
+    gl = (1.0 * 4.0 * PI * np.sin(0.0) / 1.0 + 0.0 * np.cos(0.0)) * np.sin(8 * PI * XX) * np.cos(6 * PI * YY)
+    gr = (-0.0 * 4.0 * PI * np.sin(4 * PI) / 1.0 + 1.0 * np.cos(4 * PI)) * np.sin(8 * PI * XX) * np.cos(6 * PI * YY)
+    ex = np.sin(8 * PI * X) * np.cos(6 * PI * Y) * np.cos(4 * PI * Z)
+    return d, f, gl, gr, ex
+
+
+def config1_inputs(N, size):
+    d = po.make_desc(32, 32, 33, N, size=size, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
+                     alphal=0, betal=1, alphar=0, betar=1, lam=1.0)
+    x, y, zeta = wl.grid(32, 32, po.z_grid(d))
+    f, gl, gr, u = wl.manufactured_config1(x, y, zeta, 1.0)
+    return d, f, gl, gr, u
+
+
+def config2_inputs(N, size, lam=10.0):
+    d = po.make_desc(256, 256, 129, N, size=size, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
+                     alphal=0, betal=1, alphar=0, betar=1, lam=lam)
+    x, y, zeta = wl.grid(256, 256, po.z_grid(d))
+    f, gl, gr = wl.random_problem(x, y, zeta, seed=2, nmodes=64, kmax=32, lmax=64)
+    return d, f, gl, gr
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: make -C oracle ref")
+    with open(os.path.join(HERE, "gll.json"), "w") as fp:
+        json.dump(gll_fixture(), fp, indent=0)
+    with open(os.path.join(HERE, "lu.json"), "w") as fp:
+        json.dump(lu_fixture(), fp, indent=0)
+    with open(os.path.join(HERE, "fft.json"), "w") as fp:
+        json.dump(fft_fixture(), fp, indent=0)
+    for size in (1, 4):
+        d, f, gl, gr, ex = shipped_inputs(size)
+        solve_case(f"shipped_np{size}", d, f, gl, gr, ex, gen="TestNonPeriodic3D_MPI.cpp inputs")
+    for N, size in ((33, 1), (9, 1), (9, 4)):
+        d, f, gl, gr, ex = config1_inputs(N, size)
+        solve_case(f"config1_N{N}_np{size}", d, f, gl, gr, ex, gen="manufactured_config1 lam=1")
+    for N, size in ((129, 1), (33, 4)):
+        d, f, gl, gr = config2_inputs(N, size)
+        solve_case(f"config2_N{N}_np{size}", d, f, gl, gr,
+                   gen="random_problem(seed=2, nmodes=64, kmax=32, lmax=64) lam=10")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""The C-ABI library without a device: it loads, exports every symbol that
+include/hz.h declares, and its host-side plan setup is bit-identical to the
+reference (golden vectors) -- no compute calls on a GPU."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+from golden_util import load, sha
+
+HEADER = __import__("os").path.join(__import__("os").path.dirname(__file__), "..", "include", "hz.h")
+
+
+def declared_symbols():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hz_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mpi_amd import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+    _lib.load()
+
+
+@pytest.mark.parametrize("N,ne", [(3, 1), (10, 25), (33, 1), (9, 4), (129, 1), (257, 1)])
+def test_host_plan_setup_bitexact(N, ne):
+    """hz_plan_create's GLL setup equals the reference's bit for bit (the
+    reference's D, Dzz for one element; scaling as NonPeriodic3D_MPI.cpp:125-131)."""
+    from mpi_amd import _lib
+    from oracle import pyoracle as po
+    lib = _lib.load()
+    P = ctypes.POINTER(ctypes.c_double)
+    x = np.zeros(N)
+    z = np.zeros(ne * (N - 1) + 1)
+    Dz = np.zeros((N, N))
+    Dzz = np.zeros((N, N))
+    _lib.check(lib.hz_host_plan_operators(N, ne, 2.0, x.ctypes.data_as(P), z.ctypes.data_as(P),
+                                          Dz.ctypes.data_as(P), Dzz.ctypes.data_as(P)))
+    g = load("gll.json")
+    if str(N) in g:
+        assert sha(x) == g[str(N)]["sha_x"]
+    zo, Dzo, Dzzo = po.plan_operators(N, ne, 1, 0, 2.0)
+    assert np.array_equal(z, zo) and np.array_equal(Dz, Dzo) and np.array_equal(Dzz, Dzzo)
+
+
+@pytest.mark.parametrize("N", [9, 33, 129])
+def test_host_eigendecomposition(N):
+    """Dzz_int = V diag(mu) V^-1 to round-off; eigenvalues real, negative, and
+    the lowest one tends to -(pi/2)^2 * (2/Lz)^2 (Dirichlet Laplacian on [-1,1])."""
+    from mpi_amd import _lib
+    lib = _lib.load()
+    P = ctypes.POINTER(ctypes.c_double)
+    m = N - 2
+    mu = np.zeros(m)
+    V = np.zeros((m, m))
+    Vi = np.zeros((m, m))
+    _lib.check(lib.hz_host_eigen(N, 1, 2.0, mu.ctypes.data_as(P), V.ctypes.data_as(P), Vi.ctypes.data_as(P)))
+    Dzz = np.zeros((N, N))
+    _lib.check(lib.hz_host_plan_operators(N, 1, 2.0, None, None, None, Dzz.ctypes.data_as(P)))
+    A = Dzz[1:-1, 1:-1]
+    assert np.all(mu < 0) and np.all(np.diff(mu) > 0)
+    assert np.abs(A @ V - V * mu).max() <= 1e-15 * np.abs(mu).max() * m
+    assert np.abs(V @ Vi - np.eye(m)).max() <= 1e-14
+    assert abs(mu[-1] + (math.pi / 2) ** 2) < 1e-6
+
+
+import math  # noqa: E402
