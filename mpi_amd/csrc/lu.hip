@@ -1,365 +1,411 @@
-// lu.hip -- the reference algorithm per (|kx|,|ky|) mode, one work-group per
+// lu.hip -- the reference algorithm per (|kx|,|ky|) mode: one work-group per
 // mode, the whole augmented system resident in registers.
 //
 // For mode (iw, jw) the work-group
-//   1. assembles A = Dzz_int - sigma I (or the caller's operator) and appends
-//      the right-hand sides as extra columns: the two homogeneous ones
-//      (-Dzz[1..N-2][0], -Dzz[1..N-2][N-1], NonPeriodic3D_MPI.cpp:211-219)
-//      and the interior spectrum of every element for each real column of
-//      the mode (:188-201, :248-325);
-//   2. eliminates with the reference's scaled partial pivoting
-//      (LU_Decomposition.cpp:7-72: scale of the ORIGINAL row at each position,
-//      strictly-greater test so the first maximum wins, singular test
-//      |scaled pivot| <= 1e-12, swaps of columns >= k done with the reference's
-//      add/subtract exchange).  Carrying the right-hand sides through the
-//      elimination is exactly LUSolve's forward phase (:86-95);
+//   1. assembles A = Dzz_int - sigma I (or the caller's operator) and the
+//      right-hand sides: the two homogeneous ones (-Dzz[1..N-2][0],
+//      -Dzz[1..N-2][N-1], NonPeriodic3D_MPI.cpp:211-219) and the interior
+//      spectrum of every element for each real column of the mode
+//      (:188-201, :248-325);
+//   2. eliminates with the reference's pivot rule (LU_Decomposition.cpp:7-72):
+//      the candidate at position i is |a_ik| / s_i where s_i is the max-norm
+//      of the ORIGINAL row at position i (the scale is never exchanged with
+//      the rows), the strictly-greater test makes the first maximum in
+//      position order win, |scaled pivot| <= 1e-12 is singular, and an
+//      exchange rounds both rows like the reference's a+b-b swap (matrix
+//      columns as in LUDecomposition, right-hand sides as in LUSolve).
+//      Carrying the right-hand sides through the elimination is LUSolve's
+//      forward phase (:86-95);
 //   3. back-substitutes the right-hand sides (:97-107);
 //   4. forms the end-point derivatives Dz[0|N-1].col (GLLQuadrature.cpp:234-245),
 //      runs the element-interface sweep with the Robin end rows
 //      (NonPeriodic3D_MPI.cpp:327-580) and assembles u = p + zl ul + zr ur
 //      (:582-587) straight into the spectrum.
 //
-// Layout: TR x TC threads, thread (r, c) = threadIdx r + TR*c owns rows
-// r + TR*a (a < RB) and columns c + TC*b (b < CB) of the augmented matrix.
-// The TR threads of one column group are consecutive lanes of one wave, so
-// pivot searches and back-substitution broadcasts are wave-local.
+// Exchanges are implicit: rows never move, every physical row carries its
+// current position, and the pivot row is broadcast from wherever it lives.
+// Elimination uses FMA, so results agree with the reference to round-off.
+//
+// Layout: 64*TC threads.  Wave c (of TC) owns matrix columns c + TC*b
+// (b < CB) and right-hand sides c + TC*t (t < NR); lane r owns physical rows
+// r + 64*a (a < RB).  A column lives in one wave, so the pivot search is a
+// wave reduction and each step needs one work-group barrier.  Run-time slot
+// positions are reached through jump tables (lu_slots.h), never by indexing
+// a register array.
 #include "hz_internal.h"
-
-#include <climits>
+#include "lu_slots.h"
 
 namespace hz {
 namespace {
 
 constexpr double kEps = 1.000000000000E-12;  // include/General.h:15
+constexpr int WAVE = 64;
 
-template <int WIDTH>
-__device__ __forceinline__ double shfl_d(double v, int src)
+__device__ __forceinline__ void wave_sync()
 {
-    return __shfl(v, src, WIDTH);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-template <int WIDTH>
-__device__ __forceinline__ double shfl_xor_d(double v, int m)
-{
-    return __shfl_xor(v, m, WIDTH);
-}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-enum { MODE_SOLVE = 0, MODE_FACTOR = 1 };
-
-// Elimination core shared by the mode solver and the batched factorisation.
-// ncols_mat = n (columns that belong to the matrix); columns >= n are
-// right-hand sides.  Returns the singular flag; det accumulated in *det.
-template <int TR, int TC, int RB, int CB>
-struct Core {
-    static constexpr int NT = TR * TC;
-    static constexpr int ROWS = TR * RB;
-    static constexpr int COLS = TC * CB;
-
-    struct Smem {
-        double lbuf[2][ROWS];
-        double rowk[COLS], rowp[COLS];
-        double srow[ROWS];
-        double part[TC][ROWS];
-        int p[2];
-        double det;
-        int singular;
-    };
-
-    // A[a][b] register tile.  Returns nothing; s.singular set on failure.
-    static __device__ __forceinline__ void eliminate(double (&A)[RB][CB], int n, Smem &s, int r, int c,
-                                                     int *piv_out)
-    {
-        // Row scales from the original matrix columns (< n), LU_Decomposition.cpp:20-29.
-#pragma unroll
-        for (int a = 0; a < RB; ++a) {
-            double m = 0.0;
-#pragma unroll
-            for (int b = 0; b < CB; ++b) {
-                const int j = c + TC * b;
-                if (j < n) m = fmax(m, fabs(A[a][b]));
-            }
-            s.part[c][r + TR * a] = m;
-        }
-        if (threadIdx.x == 0) {
-            s.det = 1.0;
-            s.singular = 0;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < ROWS; i += NT) {
-            double m = s.part[0][i];
-            for (int cc = 1; cc < TC; ++cc) m = fmax(m, s.part[cc][i]);
-            s.srow[i] = m;
-        }
-        __syncthreads();
-        double srow[RB];
-#pragma unroll
-        for (int a = 0; a < RB; ++a) srow[a] = s.srow[r + TR * a];
-
-        for (int k = 0; k < n - 1; ++k) {
-            const int ck = k % TC, bk = k / TC, buf = k & 1;
-            // ---- phase A: pivot search, multipliers (column-k group) ----
-            if (c == ck) {
-                double col[RB];
-#pragma unroll
-                for (int a = 0; a < RB; ++a) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int b = 0; b < CB; ++b)
-                        if (b == bk) v = A[a][b];
-                    col[a] = v;
-                }
-                double best = -1.0;
-                int bi = INT_MAX;
-#pragma unroll
-                for (int a = 0; a < RB; ++a) {
-                    const int i = r + TR * a;
-                    if (i >= k && i < n) {
-                        const double ratio = fabs(col[a]) / srow[a];
-                        if (ratio > best || (ratio == best && i < bi)) {
-                            best = ratio;
-                            bi = i;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int m = 1; m < TR; m <<= 1) {
-                    const double ob = shfl_xor_d<TR>(best, m);
-                    const int oi = __shfl_xor(bi, m, TR);
-                    if (ob > best || (ob == best && oi < bi)) {
-                        best = ob;
-                        bi = oi;
-                    }
-                }
-                if (bi == INT_MAX) {  // no comparable candidate (NaN scales): report singular
-                    bi = k;
-                    best = 0.0;
-                }
-                const int p = bi;
-                const int rk = k % TR, ak = k / TR, rp = p % TR, ap = p / TR;
-                double ck_own = 0.0, cp_own = 0.0;
-#pragma unroll
-                for (int a = 0; a < RB; ++a) {
-                    if (a == ak) ck_own = col[a];
-                    if (a == ap) cp_own = col[a];
-                }
-                const double akk = shfl_d<TR>(ck_own, rk);
-                const double apk = shfl_d<TR>(cp_own, rp);
-                double piv = akk, newpk = apk;
-                if (p != k) {
-                    const double sum = akk + apk;
-                    newpk = sum - apk;
-                    piv = sum - newpk;
-                }
-#pragma unroll
-                for (int a = 0; a < RB; ++a) {
-                    const int i = r + TR * a;
-                    double l = 0.0;
-                    if (i > k && i < n) {
-                        const double v = (i == p) ? newpk : col[a];
-                        l = v / piv;
-                    }
-                    s.lbuf[buf][i] = l;
-                    // column k of the factor: multipliers below, pivot on the diagonal
-#pragma unroll
-                    for (int b = 0; b < CB; ++b)
-                        if (b == bk) {
-                            if (i > k && i < n) A[a][b] = l;
-                            if (i == k) A[a][b] = piv;
-                        }
-                }
-                if (r == 0) {
-                    s.p[buf] = p;
-                    if (piv_out) piv_out[k] = p;
-                    double det = s.det;
-                    if (p != k) det = -det;
-                    s.det = det * piv;
-                    if (fabs(best) <= kEps) s.singular = 1;
-                }
-            }
-            __syncthreads();
-            // ---- phase B: publish rows k and p (columns > k) ----
-            const int p = s.p[buf];
-            const int rk = k % TR, ak = k / TR, rp = p % TR, ap = p / TR;
-            if (r == rk) {
-#pragma unroll
-                for (int b = 0; b < CB; ++b) {
-                    const int j = c + TC * b;
-                    double v = 0.0;
-#pragma unroll
-                    for (int a = 0; a < RB; ++a)
-                        if (a == ak) v = A[a][b];
-                    if (j > k) s.rowk[j] = v;
-                }
-            }
-            if (p != k && r == rp) {
-#pragma unroll
-                for (int b = 0; b < CB; ++b) {
-                    const int j = c + TC * b;
-                    double v = 0.0;
-#pragma unroll
-                    for (int a = 0; a < RB; ++a)
-                        if (a == ap) v = A[a][b];
-                    if (j > k) s.rowp[j] = v;
-                }
-            }
-            __syncthreads();
-            // ---- phase C: swap + rank-1 update ----
-            double P[CB], NP[CB];
-#pragma unroll
-            for (int b = 0; b < CB; ++b) {
-                const int j = c + TC * b;
-                P[b] = 0.0;
-                NP[b] = 0.0;
-                if (TC * b + TC - 1 > k && j > k) {
-                    const double vk = s.rowk[j];
-                    if (p != k) {
-                        const double vp = s.rowp[j];
-                        const double sum = vk + vp;
-                        if (j < n) {  // LUDecomposition: Swap(A[k][j], A[p][j])
-                            NP[b] = sum - vp;
-                            P[b] = sum - NP[b];
-                        } else {      // LUSolve: Swap(b[p], b[k])
-                            P[b] = sum - vk;
-                            NP[b] = sum - P[b];
-                        }
-                    } else {
-                        P[b] = vk;
-                    }
-                }
-            }
-            double Lr[RB];
-#pragma unroll
-            for (int a = 0; a < RB; ++a) Lr[a] = s.lbuf[buf][r + TR * a];
-#pragma unroll
-            for (int a = 0; a < RB; ++a) {
-                if (TR * a + TR - 1 < k) continue;  // every row of this slot is final
-#pragma unroll
-                for (int b = 0; b < CB; ++b) {
-                    if (TC * b + TC - 1 <= k) continue;  // every column of this slot is final
-                    const int i = r + TR * a, j = c + TC * b;
-                    double v = A[a][b];
-                    if (a == ap && p != k && i == p && j > k) v = NP[b];
-                    v -= Lr[a] * P[b];
-                    if (a == ak && i == k && j > k) v = P[b];
-                    A[a][b] = v;
-                }
-            }
-        }
-        // final pivot (not tested by the reference), determinant
-        __syncthreads();
-        if (n >= 1) {
-            const int k = n - 1, ck = k % TC, bk = k / TC, rk = k % TR, ak = k / TR;
-            if (c == ck && r == rk) {
-                double v = 0.0;
-#pragma unroll
-                for (int a = 0; a < RB; ++a)
-#pragma unroll
-                    for (int b = 0; b < CB; ++b)
-                        if (a == ak && b == bk) v = A[a][b];
-                s.det = s.det * v;
-            }
-        }
-        __syncthreads();
-    }
+template <int RB, int CB, int NR, int TC>
+struct Smem {
+    static constexpr int ROWS = WAVE * RB;
+    double scale[ROWS];        // s by position, LU_Decomposition.cpp:20-29
+    double lbuf[2][ROWS];      // multipliers by physical row
+    double part[TC][ROWS];
+    double prow[TC][CB];       // per-wave staging of the pivot row
+    double krow[TC][CB];       // ... and of the row it is exchanged with
+    double prowy[TC][NR];
+    double krowy[TC][NR];
+    int info[2][3];            // {npivot[k], physical pivot row, physical row at position k}
+    int phys_of_pos[ROWS];
+    double udiag[ROWS];        // U diagonal by position
+    double det;
+    int singular;
 };
 
-// Mode solver: see the file comment.  NRHS_MAX = CB*TC - n columns available.
-template <int TR, int TC, int RB, int CB>
-__global__ __launch_bounds__(TR *TC) void lu_modes_kernel(const LuModeArgs a)
+// Elimination core.  A[a][b]: physical row lane + 64a, matrix column
+// c + TC*b.  Y[a][t]: same row, right-hand side c + TC*t (nrhs of them).
+// pos[a]: current position of the row.  With out != nullptr the factors are
+// written in the reference's storage (multipliers at their positions, U rows
+// in position order) and the pivots to piv_out.
+template <int RB, int CB, int NR, int TC>
+__device__ __forceinline__ void eliminate(double (&A)[RB][CB], double (&Y)[RB][NR], int (&pos)[RB], int n,
+                                          int nrhs, Smem<RB, CB, NR, TC> &s, double *out, int *piv_out)
 {
-    using C = Core<TR, TC, RB, CB>;
-    constexpr int ROWS = C::ROWS;
-    __shared__ typename C::Smem sm;
-    __shared__ double ubuf[2][TC][ROWS];
-    __shared__ double sh_ul[ROWS], sh_ur[ROWS];
-    __shared__ double dots[2][128];  // [0|N-1][rhs index]
-    __shared__ double zeta[2][128];  // [left|right][element*4 + real column]
+    const int lane = threadIdx.x % WAVE, c = uniform(threadIdx.x / WAVE);
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        double m = 0.0;
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+            if (c + TC * b < n) m = fmax(m, fabs(A[a][b]));
+        s.part[c][lane + WAVE * a] = m;
+        pos[a] = lane + WAVE * a;
+    }
+    if (threadIdx.x == 0) {
+        s.det = 1.0;
+        s.singular = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < WAVE * RB; i += WAVE * TC) {
+        double m = s.part[0][i];
+#pragma unroll
+        for (int cc = 1; cc < TC; ++cc) m = fmax(m, s.part[cc][i]);
+        s.scale[i] = m;
+    }
+    __syncthreads();
 
-    const int tid = threadIdx.x, r = tid % TR, c = tid / TR;
+    for (int k = 0; k < n - 1; ++k) {
+        const int buf = k & 1;
+        const int ck = k % TC, bk = k / TC;  // uniform
+        // ---------------- phase A: pivot search in the wave owning column k
+        if (c == ck) {
+            double colv[RB];
+#pragma unroll
+            for (int a = 0; a < RB; ++a) colv[a] = 0.0;
+#define HZ_GETCOL(b)                                                            \
+    if constexpr ((b) < CB) {                                                   \
+        _Pragma("unroll") for (int a = 0; a < RB; ++a) colv[a] = A[a][(b)];     \
+    }
+            HZ_SLOT_AT(bk, HZ_GETCOL)
+#undef HZ_GETCOL
+            double best = -2.0;
+            int bpos = 0x7fffffff, brow = 0, mine = -1;
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                const int row = lane + WAVE * a;
+                if (row < n && pos[a] >= k) {
+                    if (pos[a] == k) mine = row;
+                    double ratio = fabs(colv[a]) / s.scale[pos[a]];
+                    if (!(ratio >= 0.0)) ratio = -1.0;  // NaN scale: lowest priority
+                    if (ratio > best || (ratio == best && pos[a] < bpos)) {
+                        best = ratio;
+                        bpos = pos[a];
+                        brow = row;
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 1; m < WAVE; m <<= 1) {
+                const double ob = __shfl_xor(best, m);
+                const int op = __shfl_xor(bpos, m);
+                const int orw = __shfl_xor(brow, m);
+                if (ob > best || (ob == best && op < bpos)) {
+                    best = ob;
+                    bpos = op;
+                    brow = orw;
+                }
+            }
+            const int ppos = uniform(bpos), ph = uniform(brow);
+            const int phk = uniform(__builtin_amdgcn_readlane(mine, __builtin_ctzll(__ballot(mine >= 0))));
+            double own = colv[0], ownk = colv[0];
+#pragma unroll
+            for (int a = 1; a < RB; ++a) {
+                if (a == ph / WAVE) own = colv[a];
+                if (a == phk / WAVE) ownk = colv[a];
+            }
+            const double bkv = __shfl(own, ph % WAVE);    // pivot row, column k
+            const double akv = __shfl(ownk, phk % WAVE);  // row at position k, column k
+            double piv = bkv, akswap = akv;
+            if (ppos != k) {  // Swap(A[k][k], A[p][k])
+                const double sum = akv + bkv;
+                akswap = sum - bkv;
+                piv = sum - akswap;
+            }
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                const int row = lane + WAVE * a;
+                int np = pos[a];
+                if (row == ph) np = k;
+                else if (np == k) np = ppos;
+                const bool active = row < n && np > k;
+                const double v = (row == phk && ppos != k) ? akswap : colv[a];
+                const double l = active ? v / piv : 0.0;
+                s.lbuf[buf][row] = l;
+                if (out) {
+                    if (active) out[(long long)np * n + k] = l;
+                    if (row == ph) out[(long long)k * n + k] = piv;
+                }
+            }
+            if (lane == 0) {
+                s.udiag[k] = piv;
+                s.info[buf][0] = ppos;
+                s.info[buf][1] = ph;
+                s.info[buf][2] = phk;
+                s.phys_of_pos[k] = ph;
+                if (piv_out) piv_out[k] = ppos;
+                double det = s.det;
+                if (ppos != k) det = -det;
+                s.det = det * piv;
+                if (fabs(best) <= kEps || best < 0.0) s.singular = 1;
+            }
+        }
+        __syncthreads();
+        // ---------------- phase C: exchange positions, rank-1 update
+        const int ppos = uniform(s.info[buf][0]), ph = uniform(s.info[buf][1]);
+        const int phk = uniform(s.info[buf][2]);
+        const bool swapped = ppos != k;
+        double l[RB];
+#pragma unroll
+        for (int a = 0; a < RB; ++a) {
+            const int row = lane + WAVE * a;
+            if (row == ph) pos[a] = k;
+            else if (pos[a] == k) pos[a] = ppos;
+            l[a] = s.lbuf[buf][row];
+        }
+        const int b0 = (c > ck) ? bk : bk + 1;  // first slot of this wave right of k
+        const int phslot = ph / WAVE, phlane = ph % WAVE;
+        const int kslot = phk / WAVE, klane = phk % WAVE;
+        // stage the pivot row (and, on an exchange, the row it replaces)
+#pragma unroll
+        for (int aa = 0; aa < RB; ++aa) {
+            if (aa == phslot && lane == phlane) {
+#define HZ_STAGE(b)                                                             \
+    if constexpr ((b) < CB) s.prow[c][(b)] = A[aa][(b)];
+                HZ_SLOTS_FROM(b0, HZ_STAGE)
+#undef HZ_STAGE
+#pragma unroll
+                for (int t = 0; t < NR; ++t) s.prowy[c][t] = Y[aa][t];
+            }
+            if (swapped && aa == kslot && lane == klane) {
+#define HZ_STAGEK(b)                                                            \
+    if constexpr ((b) < CB) s.krow[c][(b)] = A[aa][(b)];
+                HZ_SLOTS_FROM(b0, HZ_STAGEK)
+#undef HZ_STAGEK
+#pragma unroll
+                for (int t = 0; t < NR; ++t) s.krowy[c][t] = Y[aa][t];
+            }
+        }
+        wave_sync();
+        if (swapped) {
+            // The reference exchanges rows k and p with a+b-b arithmetic; the
+            // two owner lanes apply the same roundings to their rows.
+            if (lane == phlane || lane == klane) {
+#define HZ_XCHG(b)                                                              \
+    if constexpr ((b) < CB) {                                                   \
+        const double bv = s.prow[c][(b)], av = s.krow[c][(b)];                  \
+        const double sum = av + bv;                                             \
+        const double kv = sum - bv; /* LUDecomposition: Swap(A[k][j], A[p][j]) */ \
+        const double pv = sum - kv;                                             \
+        _Pragma("unroll") for (int a = 0; a < RB; ++a) {                        \
+            if (a == kslot && lane == klane) A[a][(b)] = kv;                    \
+            if (a == phslot && lane == phlane) A[a][(b)] = pv;                  \
+        }                                                                       \
+        if (lane == phlane) s.prow[c][(b)] = pv;                                \
+    }
+                HZ_SLOTS_FROM(b0, HZ_XCHG)
+#undef HZ_XCHG
+#pragma unroll
+                for (int t = 0; t < NR; ++t) {
+                    const double bv = s.prowy[c][t], av = s.krowy[c][t];
+                    const double sum = av + bv;
+                    const double pv = sum - av;  // LUSolve: Swap(b[p], b[k])
+                    const double kv = sum - pv;
+#pragma unroll
+                    for (int a = 0; a < RB; ++a) {
+                        if (a == kslot && lane == klane) Y[a][t] = kv;
+                        if (a == phslot && lane == phlane) Y[a][t] = pv;
+                    }
+                    if (lane == phlane) s.prowy[c][t] = pv;
+                }
+            }
+            wave_sync();
+        }
+        if (out && lane == 0) {
+            for (int b = b0; b < CB; ++b) {
+                const int j = c + TC * b;
+                if (j < n) out[(long long)k * n + j] = s.prow[c][b];
+            }
+        }
+#define HZ_UPDATE(b)                                                            \
+    if constexpr ((b) < CB) {                                                   \
+        const double pv = s.prow[c][(b)];                                       \
+        _Pragma("unroll") for (int a = 0; a < RB; ++a) A[a][(b)] = fma(-l[a], pv, A[a][(b)]); \
+    }
+        HZ_SLOTS_FROM(b0, HZ_UPDATE)
+#undef HZ_UPDATE
+        if (nrhs > 0) {
+#pragma unroll
+            for (int t = 0; t < NR; ++t) {
+                const double pv = s.prowy[c][t];
+#pragma unroll
+                for (int a = 0; a < RB; ++a) Y[a][t] = fma(-l[a], pv, Y[a][t]);
+            }
+        }
+        wave_sync();
+    }
+    // last position: its row is the last pivot (not tested by the reference)
+    __syncthreads();
+    {
+        const int k = n - 1, ck = k % TC, bk = k / TC;
+        if (c == ck) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                const int row = lane + WAVE * a;
+                if (row < n && pos[a] == k) {
+                    double v = 0.0;
+#define HZ_GETV(b)                                                              \
+    if constexpr ((b) < CB) v = A[a][(b)];
+                    HZ_SLOT_AT(bk, HZ_GETV)
+#undef HZ_GETV
+                    s.udiag[k] = v;
+                    s.det = s.det * v;
+                    s.phys_of_pos[k] = row;
+                    if (out) out[(long long)k * n + k] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+template <int RB, int CB, int NR, int TC>
+__global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
+{
+    constexpr int ROWS = WAVE * RB;
+    constexpr int BQ = 4 * TC;  // back-substitution block of U columns
+    __shared__ Smem<RB, CB, NR, TC> sm;
+    __shared__ double ubuf[2][BQ][ROWS];
+    __shared__ double sh_ul[ROWS], sh_ur[ROWS];
+    __shared__ double dots[2][128];
+    __shared__ double zeta[2][128];
+
+    const int lane = threadIdx.x % WAVE, c = uniform(threadIdx.x / WAVE);
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
     const int mode = blockIdx.x;
     const int iw = mode / M, jw = mode % M;
-    // real columns of this mode: (x index, re/im)
-    int xi[2];
-    int nxi = 1;
-    xi[0] = iw;
-    if (iw != 0 && iw != a.nx / 2) {
-        xi[1] = a.nx - iw;
-        nxi = 2;
-    }
+    const int xi0 = iw, xi1 = a.nx - iw, nxi = (iw != 0 && iw != a.nx / 2) ? 2 : 1;
     const int nc = 2 * nxi;
     const int nrhs = 2 + 4 * E;
 
-    // ---- assemble the augmented matrix ----
-    double A[RB][CB];
+    // ---- assemble (physical row i = position i) ----
+    double A[RB][CB], Y[RB][NR];
+    int pos[RB];
     const double sig = a.shift ? a.shift[mode] : 0.0;
 #pragma unroll
     for (int aa = 0; aa < RB; ++aa) {
-        const int i = r + TR * aa;
+        const int i = lane + WAVE * aa;
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
             const int j = c + TC * b;
             double v = 0.0;
-            if (i < n) {
-                if (j < n) {
-                    if (a.op_full) {
-                        v = a.op_full[((long long)mode * n + i) * n + j];
-                    } else {
-                        v = a.DzzT[(long long)j * n + i];
-                        if (i == j) v = a.op_diag ? a.op_diag[(long long)mode * n + i] : v - sig;
-                    }
-                } else if (j < n + nrhs) {
-                    const int t = j - n;
-                    if (t == 0) {
-                        v = -a.Dzz[(long long)(i + 1) * N];
-                    } else if (t == 1) {
-                        v = -a.Dzz[(long long)(i + 1) * N + (N - 1)];
-                    } else {
-                        const int e = (t - 2) >> 2, cc = (t - 2) & 3;
-                        if (cc < nc) {
-                            const long long col = ((long long)xi[cc >> 1] * M + jw) * K;
-                            const double *src = (cc & 1) ? a.spec_im : a.spec_re;
-                            v = src[col + (long long)e * (N - 1) + 1 + i];
-                        }
-                    }
+            if (i < n && j < n) {
+                if (a.op_full) {
+                    v = a.op_full[((long long)mode * n + i) * n + j];
+                } else {
+                    v = a.DzzT[(long long)j * n + i];
+                    if (i == j) v = a.op_diag ? a.op_diag[(long long)mode * n + i] : v - sig;
                 }
             }
             A[aa][b] = v;
         }
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const int r = c + TC * t;  // right-hand-side index
+            double v = 0.0;
+            if (i < n && r < nrhs) {
+                if (r == 0) {
+                    v = -a.Dzz[(long long)(i + 1) * N];
+                } else if (r == 1) {
+                    v = -a.Dzz[(long long)(i + 1) * N + (N - 1)];
+                } else {
+                    const int e = (r - 2) >> 2, cc = (r - 2) & 3;
+                    if (cc < nc) {
+                        const int xi = (cc >> 1) ? xi1 : xi0;
+                        const long long col = ((long long)xi * M + jw) * K;
+                        const double *src = (cc & 1) ? a.spec_im : a.spec_re;
+                        v = src[col + (long long)e * (N - 1) + 1 + i];
+                    }
+                }
+            }
+            Y[aa][t] = v;
+        }
     }
 
-    C::eliminate(A, n, sm, r, c, nullptr);
+    eliminate<RB, CB, NR, TC>(A, Y, pos, n, nrhs, sm, nullptr, nullptr);
 
-    // ---- back substitution of the right-hand-side columns ----
-    const int nblk = (n + TC - 1) / TC;
+    // ---- back substitution, U staged through LDS in blocks of BQ columns ----
+    const int nblk = (n + BQ - 1) / BQ;
     int buf = 0;
     for (int blk = nblk - 1; blk >= 0; --blk) {
-#pragma unroll
-        for (int b = 0; b < CB; ++b)
-            if (b == blk) {
-#pragma unroll
-                for (int aa = 0; aa < RB; ++aa) ubuf[buf][c][r + TR * aa] = A[aa][b];
-            }
+        const int q0 = blk * BQ;
+        // wave c owns the block's columns q0 + c + TC*t, i.e. slots blk*BQ/TC + t
+#define HZ_UWRITE(b)                                                            \
+    if constexpr ((b) < CB) {                                                   \
+        const int j = c + TC * (b);                                             \
+        if (j < q0 + BQ) {                                                      \
+            _Pragma("unroll") for (int aa = 0; aa < RB; ++aa) ubuf[buf][j - q0][lane + WAVE * aa] = A[aa][(b)]; \
+        }                                                                       \
+    }
+        {
+            const int bs = blk * (BQ / TC);
+            HZ_SLOTS_FROM(bs, HZ_UWRITE)
+        }
+#undef HZ_UWRITE
         __syncthreads();
-        const int q_hi = min(n - 1, TC * blk + TC - 1), q_lo = TC * blk;
+        const int q_hi = min(n - 1, q0 + BQ - 1);
+        for (int q = q_hi; q >= q0; --q) {
+            const int ph = uniform(sm.phys_of_pos[q]);
+            const double uqq = sm.udiag[q];
 #pragma unroll
-        for (int b = 0; b < CB; ++b) {
-            const int j = c + TC * b;
-            if (TC * b + TC - 1 < n) continue;          // slot holds no right-hand side
-            if (!(j >= n && j < n + nrhs)) continue;    // this lane's column is not one
-            for (int q = q_hi; q >= q_lo; --q) {
-                const int rq = q % TR, aq = q / TR;
-                double own = 0.0;
+            for (int t = 0; t < NR; ++t) {
+                if (c + TC * t >= nrhs) continue;  // uniform
+                double own = Y[0][t];
 #pragma unroll
-                for (int aa = 0; aa < RB; ++aa)
-                    if (aa == aq) own = A[aa][b];
-                const double yq = shfl_d<TR>(own, rq);
-                const double xq = yq / ubuf[buf][q - q_lo][q];
+                for (int aa = 1; aa < RB; ++aa)
+                    if (aa == ph / WAVE) own = Y[aa][t];
+                const double xq = __shfl(own, ph % WAVE) / uqq;
 #pragma unroll
                 for (int aa = 0; aa < RB; ++aa) {
-                    const int row = r + TR * aa;
-                    if (row < q) A[aa][b] -= ubuf[buf][q - q_lo][row] * xq;
-                    if (aa == aq && r == rq) A[aa][b] = xq;
+                    const int row = lane + WAVE * aa;
+                    double v = Y[aa][t];
+                    if (pos[aa] < q && row < n) v = fma(-ubuf[buf][q - q0][row], xq, v);
+                    if (row == ph) v = xq;
+                    Y[aa][t] = v;
                 }
             }
         }
@@ -367,74 +413,73 @@ __global__ __launch_bounds__(TR *TC) void lu_modes_kernel(const LuModeArgs a)
     }
     __syncthreads();
 
-    // ---- end-point derivatives of every right-hand-side column ----
+    // ---- end-point derivatives of every right-hand side ----
 #pragma unroll
-    for (int b = 0; b < CB; ++b) {
-        const int j = c + TC * b;
-        if (TC * b + TC - 1 < n) continue;
-        const bool mine = (j >= n && j < n + nrhs);
+    for (int t = 0; t < NR; ++t) {
+        const int r = c + TC * t;
+        if (r >= nrhs) continue;  // uniform
         double d0 = 0.0, dN = 0.0;
-        if (mine) {
 #pragma unroll
-            for (int aa = 0; aa < RB; ++aa) {
-                const int row = r + TR * aa;
-                if (row < n) {
-                    d0 += a.Dz[row + 1] * A[aa][b];
-                    dN += a.Dz[(long long)(N - 1) * N + row + 1] * A[aa][b];
-                }
+        for (int aa = 0; aa < RB; ++aa) {
+            const int row = lane + WAVE * aa;
+            if (row < n) {
+                const int q = pos[aa];
+                d0 = fma(a.Dz[q + 1], Y[aa][t], d0);
+                dN = fma(a.Dz[(long long)(N - 1) * N + q + 1], Y[aa][t], dN);
+                if (r == 0) sh_ul[q] = Y[aa][t];
+                if (r == 1) sh_ur[q] = Y[aa][t];
             }
         }
 #pragma unroll
-        for (int m = 1; m < TR; m <<= 1) {
-            d0 += shfl_xor_d<TR>(d0, m);
-            dN += shfl_xor_d<TR>(dN, m);
+        for (int m = 1; m < WAVE; m <<= 1) {
+            d0 += __shfl_xor(d0, m);
+            dN += __shfl_xor(dN, m);
         }
-        if (mine) {
-            const int t = j - n;
-            if (t == 0) {  // ul has u(0) = 1
-                d0 += a.Dz[0];
-                dN += a.Dz[(long long)(N - 1) * N];
-            } else if (t == 1) {  // ur has u(N-1) = 1
-                d0 += a.Dz[N - 1];
-                dN += a.Dz[(long long)(N - 1) * N + N - 1];
-            }
-            if (r == 0 && t < 128) {
-                dots[0][t] = d0;
-                dots[1][t] = dN;
-            }
-#pragma unroll
-            for (int aa = 0; aa < RB; ++aa) {
-                const int row = r + TR * aa;
-                if (t == 0) sh_ul[row] = A[aa][b];
-                if (t == 1) sh_ur[row] = A[aa][b];
-            }
+        if (r == 0) {  // ul has u(0) = 1
+            d0 += a.Dz[0];
+            dN += a.Dz[(long long)(N - 1) * N];
+        } else if (r == 1) {  // ur has u(N-1) = 1
+            d0 += a.Dz[N - 1];
+            dN += a.Dz[(long long)(N - 1) * N + N - 1];
+        }
+        if (lane == 0 && r < 128) {
+            dots[0][r] = d0;
+            dots[1][r] = dN;
         }
     }
     __syncthreads();
 
     // ---- element-interface sweep, one thread per real column ----
-    if (tid < nc) {
-        const int cc = tid;
-        const long long bcol = (long long)xi[cc >> 1] * M + jw;
+    if (threadIdx.x < nc) {
+        const int cc = threadIdx.x;
+        const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
         const double *bsrc = (cc & 1) ? a.bc_im : a.bc_re;
         const double gl = bsrc[bcol], gr = bsrc[(long long)a.nx * M + bcol];
         const double d0ul = dots[0][0], dNul = dots[1][0], d0ur = dots[0][1], dNur = dots[1][1];
-        double bb[32], cv[32], rh[32];
-        bb[0] = -a.alpha_l * d0ul + a.beta_l;
-        cv[0] = -a.alpha_l * d0ur;
-        rh[0] = gl + a.alpha_l * dots[0][2 + cc];
+        // forward elimination over the elements; b and rhs kept in zeta
+        double bprev = -a.alpha_l * d0ul + a.beta_l;
+        double cprev = -a.alpha_l * d0ur;
+        double rprev = gl + a.alpha_l * dots[0][2 + cc];
+        zeta[0][cc] = bprev;
+        zeta[1][cc] = rprev;
         for (int e = 1; e < E; ++e) {
             const double lb = dNul;
-            bb[e] = dNur - d0ul - (cv[e - 1] / bb[e - 1]) * lb;
-            cv[e] = -d0ur;
-            rh[e] = dots[0][2 + 4 * e + cc] - dots[1][2 + 4 * (e - 1) + cc] - (rh[e - 1] / bb[e - 1]) * lb;
+            const double be = dNur - d0ul - (cprev / bprev) * lb;
+            const double re = dots[0][2 + 4 * e + cc] - dots[1][2 + 4 * (e - 1) + cc] - (rprev / bprev) * lb;
+            bprev = be;
+            cprev = -d0ur;
+            rprev = re;
+            zeta[0][4 * e + cc] = be;
+            zeta[1][4 * e + cc] = re;
         }
         const double lb = a.alpha_r * dNul;
-        const double br = a.alpha_r * dNur + a.beta_r - (cv[E - 1] / bb[E - 1]) * lb;
-        const double rhsr = gr - a.alpha_r * dots[1][2 + 4 * (E - 1) + cc] - (rh[E - 1] / bb[E - 1]) * lb;
+        const double br = a.alpha_r * dNur + a.beta_r - (cprev / bprev) * lb;
+        const double rhsr = gr - a.alpha_r * dots[1][2 + 4 * (E - 1) + cc] - (rprev / bprev) * lb;
         double zr = rhsr / br;
         for (int e = E - 1; e >= 0; --e) {
-            const double zl = (rh[e] - cv[e] * zr) / bb[e];
+            const double be = zeta[0][4 * e + cc], re = zeta[1][4 * e + cc];
+            const double ce = (e == 0) ? -a.alpha_l * d0ur : -d0ur;
+            const double zl = (re - ce * zr) / be;
             zeta[0][4 * e + cc] = zl;
             zeta[1][4 * e + cc] = zr;
             zr = zl;
@@ -444,54 +489,53 @@ __global__ __launch_bounds__(TR *TC) void lu_modes_kernel(const LuModeArgs a)
 
     // ---- assembly straight into the spectrum ----
 #pragma unroll
-    for (int b = 0; b < CB; ++b) {
-        const int j = c + TC * b;
-        if (TC * b + TC - 1 < n) continue;
-        if (!(j >= n + 2 && j < n + nrhs)) continue;
-        const int t = j - n, e = (t - 2) >> 2, cc = (t - 2) & 3;
+    for (int t = 0; t < NR; ++t) {
+        const int r = c + TC * t;
+        if (r < 2 || r >= nrhs) continue;
+        const int e = (r - 2) >> 2, cc = (r - 2) & 3;
         if (cc >= nc) continue;
         const double zl = zeta[0][4 * e + cc], zr = zeta[1][4 * e + cc];
-        const long long col = ((long long)xi[cc >> 1] * M + jw) * K + (long long)e * (N - 1);
+        const int xi = (cc >> 1) ? xi1 : xi0;
+        const long long col = ((long long)xi * M + jw) * K + (long long)e * (N - 1);
         double *dst = (cc & 1) ? a.spec_im : a.spec_re;
 #pragma unroll
         for (int aa = 0; aa < RB; ++aa) {
-            const int row = r + TR * aa;
-            if (row < n) dst[col + 1 + row] = A[aa][b] + (zl * sh_ul[row] + zr * sh_ur[row]);
+            const int row = lane + WAVE * aa;
+            if (row < n) {
+                const int q = pos[aa];
+                dst[col + 1 + q] = Y[aa][t] + (zl * sh_ul[q] + zr * sh_ur[q]);
+            }
         }
-        if (r == 0) {
+        if (lane == 0) {
             dst[col] = zl;
             dst[col + N - 1] = zr;
         }
     }
-    if (tid == 0) a.status[mode] = sm.singular ? HZ_ERR_SINGULAR : 0;
+    if (threadIdx.x == 0) a.status[mode] = sm.singular ? HZ_ERR_SINGULAR : 0;
 }
 
 // Batched factorisation in the reference's storage (no right-hand sides).
-template <int TR, int TC, int RB, int CB>
-__global__ __launch_bounds__(TR *TC) void lu_factor_kernel(const LuFactorArgs a)
+template <int RB, int CB, int TC>
+__global__ __launch_bounds__(WAVE *TC) void lu_factor_kernel(const LuFactorArgs a)
 {
-    using C = Core<TR, TC, RB, CB>;
-    __shared__ typename C::Smem sm;
-    const int tid = threadIdx.x, r = tid % TR, c = tid / TR;
+    __shared__ Smem<RB, CB, 1, TC> sm;
+    const int lane = threadIdx.x % WAVE, c = uniform(threadIdx.x / WAVE);
     const int n = a.n;
     double *Am = a.A + (long long)blockIdx.x * n * n;
-    double A[RB][CB];
+    double A[RB][CB], Y[RB][1];
+    int pos[RB];
 #pragma unroll
-    for (int aa = 0; aa < RB; ++aa)
+    for (int aa = 0; aa < RB; ++aa) {
+        Y[aa][0] = 0.0;
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
-            const int i = r + TR * aa, j = c + TC * b;
+            const int i = lane + WAVE * aa, j = c + TC * b;
             A[aa][b] = (i < n && j < n) ? Am[(long long)i * n + j] : 0.0;
         }
-    C::eliminate(A, n, sm, r, c, a.piv ? a.piv + (long long)blockIdx.x * (n - 1) : nullptr);
-#pragma unroll
-    for (int aa = 0; aa < RB; ++aa)
-#pragma unroll
-        for (int b = 0; b < CB; ++b) {
-            const int i = r + TR * aa, j = c + TC * b;
-            if (i < n && j < n) Am[(long long)i * n + j] = A[aa][b];
-        }
-    if (tid == 0) {
+    }
+    __syncthreads();  // every read of Am before the factors overwrite it
+    eliminate<RB, CB, 1, TC>(A, Y, pos, n, 0, sm, Am, a.piv ? a.piv + (long long)blockIdx.x * (n - 1) : nullptr);
+    if (threadIdx.x == 0) {
         if (a.det) a.det[blockIdx.x] = sm.det;
         if (a.status) a.status[blockIdx.x] = sm.singular ? HZ_ERR_SINGULAR : 0;
     }
@@ -526,50 +570,53 @@ __global__ void lu_solve_kernel(const double *LU, const int *piv, double *b, int
     }
 }
 
-template <int TR, int TC, int RB, int CB>
+template <int RB, int CB, int NR>
 int launch_modes_t(const LuModeArgs &a, int modes, hipStream_t s)
 {
-    hipLaunchKernelGGL((lu_modes_kernel<TR, TC, RB, CB>), dim3(modes), dim3(TR * TC), 0, s, a);
+    hipLaunchKernelGGL((lu_modes_kernel<RB, CB, NR, 4>), dim3(modes), dim3(WAVE * 4), 0, s, a);
     HZ_HIP(hipGetLastError());
     return HZ_OK;
 }
 
-template <int TR, int TC, int RB, int CB>
+template <int RB, int CB>
 int launch_factor_t(const LuFactorArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((lu_factor_kernel<TR, TC, RB, CB>), dim3(a.count), dim3(TR * TC), 0, s, a);
+    hipLaunchKernelGGL((lu_factor_kernel<RB, CB, 4>), dim3(a.count), dim3(WAVE * 4), 0, s, a);
     HZ_HIP(hipGetLastError());
     return HZ_OK;
+}
+
+template <int RB, int CB>
+int launch_modes_nr(const LuModeArgs &a, int modes, hipStream_t s)
+{
+    const int nr = (2 + 4 * a.ne + 3) / 4;  // right-hand sides per wave
+    if (nr <= 2) return launch_modes_t<RB, CB, 2>(a, modes, s);
+    if (nr <= 8) return launch_modes_t<RB, CB, 8>(a, modes, s);
+    if constexpr (RB == 1) {
+        if (nr <= 32) return launch_modes_t<1, CB, 32>(a, modes, s);
+    }
+    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: at most 7 elements when N-2 > 64");
 }
 
 }  // namespace
 
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s)
 {
-    const int n = a.n, cols = a.n + 2 + 4 * a.ne;
     if (a.ne > 31) return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form supports at most 31 elements");
-    if (n <= 8 && cols <= 32) return launch_modes_t<8, 8, 1, 4>(a, modes, s);
-    if (n <= 8 && cols <= 128) return launch_modes_t<8, 8, 1, 16>(a, modes, s);
-    if (n <= 16 && cols <= 32) return launch_modes_t<8, 8, 2, 4>(a, modes, s);
-    if (n <= 16 && cols <= 128) return launch_modes_t<8, 8, 2, 16>(a, modes, s);
-    if (n <= 32 && cols <= 48) return launch_modes_t<8, 8, 4, 6>(a, modes, s);
-    if (n <= 32 && cols <= 128) return launch_modes_t<8, 8, 4, 16>(a, modes, s);
-    if (n <= 64 && cols <= 80) return launch_modes_t<16, 16, 4, 5>(a, modes, s);
-    if (n <= 64 && cols <= 128) return launch_modes_t<16, 16, 4, 8>(a, modes, s);
-    if (n <= 128 && cols <= 144) return launch_modes_t<16, 16, 8, 9>(a, modes, s);
-    if (n <= 128 && cols <= 192) return launch_modes_t<16, 16, 8, 12>(a, modes, s);
-    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: N-2 + 2 + 4*elements must fit 192 columns (N-2 <= 128)");
+    if (a.n <= 16) return launch_modes_nr<1, 4>(a, modes, s);
+    if (a.n <= 32) return launch_modes_nr<1, 8>(a, modes, s);
+    if (a.n <= 64) return launch_modes_nr<1, 16>(a, modes, s);
+    if (a.n <= 128) return launch_modes_nr<2, 32>(a, modes, s);
+    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: N-2 must be <= 128");
 }
 
 int launch_lu_factor(const LuFactorArgs &a, hipStream_t s)
 {
     if (a.count <= 0) return HZ_OK;
     const int n = a.n;
-    if (n <= 8) return launch_factor_t<8, 8, 1, 1>(a, s);
-    if (n <= 16) return launch_factor_t<8, 8, 2, 2>(a, s);
-    if (n <= 32) return launch_factor_t<8, 8, 4, 4>(a, s);
-    if (n <= 64) return launch_factor_t<16, 16, 4, 4>(a, s);
-    if (n <= 128) return launch_factor_t<16, 16, 8, 8>(a, s);
+    if (n <= 32) return launch_factor_t<1, 8>(a, s);
+    if (n <= 64) return launch_factor_t<1, 16>(a, s);
+    if (n <= 128) return launch_factor_t<2, 32>(a, s);
     return set_error(HZ_ERR_UNSUPPORTED, "batched FP64 LU supports n <= 128");
 }
 
