@@ -1,0 +1,281 @@
+// diag.hip -- the z-diagonalised mode solve on FP64 matrix cores.
+//
+// The interior GLL operator of every element is the same matrix, so with
+// Dzz_int = V diag(mu) V^{-1} (computed once per plan, eig.cpp) the
+// reference's per-mode solve A(sigma)^{-1} = V (diag(mu) - sigma)^{-1} V^{-1}
+// needs no factorisation.  For each real spectral column (x index, y index,
+// re/im) and element e, with sigma = lambda + kx^2 + ky^2:
+//   y   = V^{-1} f_e                                  (MFMA contraction)
+//   inv = 1 / (mu - sigma),   q = y * inv
+//   p_e = V q  is the particular solution, ul = V (w0 * inv), ur = V (wN * inv)
+//   with w0 = V^{-1}(-Dzz[1..N-2][0]), wN = V^{-1}(-Dzz[1..N-2][N-1])
+//   (NonPeriodic3D_MPI.cpp:188-222); the end-point derivatives are
+//   Dz[0|N-1].p_e = r0|rN . q with r0|rN = Dz[0|N-1][1..N-2] V (:175-179);
+//   the element-interface sweep (:327-580) gives zl_e, zr_e and
+//   u_e = V ((y + zl_e w0 + zr_e wN) * inv)          (MFMA contraction)
+//   with u at the element ends equal to zl_e, zr_e (:582-587).
+//
+// A work-group owns RT tile rows (spectral column x element) and all NP
+// (padded n) eigen/grid indices.  Contractions are transposed so that MFMA
+// output lanes run along z, which is contiguous in memory:
+//   Y^T = F^T Vinv^T,   U^T = Z^T V^T
+// v_mfma_f64_16x16x4f64 operand layout (wave64, measured on gfx950):
+// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] (4 per lane) in
+// lane j + 16*(i%4), element i/4.  Each wave computes NJ/4 output column tiles for all RT rows,
+// reading A from the LDS tile and B (Vinv^T, V^T) from L2.
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int NJ, int RT, bool GENERAL>
+__global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
+{
+    constexpr int NP = 16 * NJ;      // padded interior size
+    constexpr int LD = NP + 2;       // LDS row stride (doubles)
+    constexpr int JT = NJ / 4;       // output column tiles per wave
+    constexpr int RBK = RT / 16;     // row blocks
+    __shared__ double tile[RT * LD];
+    __shared__ double row_sig[RT], row_zl[RT], row_zr[RT], row_d0[RT], row_dN[RT];
+    __shared__ long long row_col[RT];  // global offset of the row's column start (element start)
+
+    const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
+    const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
+    const int rows_per_fc = 2 * E;                    // re/im x elements
+    const int fc_per_tile = RT / rows_per_fc;
+    const int fc0 = blockIdx.x * fc_per_tile;
+    const int nfc = a.nx * M;
+
+    // ---- row bookkeeping: row = (fc_local*2 + plane)*E + e ----
+    for (int row = tid; row < RT; row += 256) {
+        const int fl = row / rows_per_fc, rem = row % rows_per_fc;
+        const int plane = rem / E, e = rem % E;
+        const int fc = fc0 + fl;
+        double sig = 0.0;
+        long long col = -1;
+        if (fl < fc_per_tile && fc < nfc) {
+            const int i = fc / M, m = fc % M;
+            const int iw = i <= a.nx / 2 ? i : a.nx - i;
+            sig = a.shift[iw * M + m];
+            col = (long long)fc * K + (long long)e * (N - 1) + (plane ? a.plane_off : 0);
+        }
+        row_sig[row] = sig;
+        row_col[row] = col;
+    }
+    __syncthreads();
+    // ---- stage F^T: tile[row][k] = f interior of the row's element ----
+    for (int idx = tid; idx < RT * NP; idx += 256) {
+        const int row = idx / NP, k = idx % NP;
+        const long long col = row_col[row];
+        double v = 0.0;
+        if (col >= 0 && k < n) v = a.spec[col + 1 + k];
+        tile[row * LD + k] = v;
+    }
+    __syncthreads();
+
+    // ---- Y^T = F^T Vinv^T ----
+    d4 acc[RBK][JT];
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
+    const int li = lane % 16, lk = lane / 16;
+    for (int ks = 0; ks < NP / 4; ++ks) {
+        const int kk = 4 * ks + lk;
+        double bv[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VinvT[(long long)kk * NP + 16 * (w * JT + jt) + li];
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb) {
+            const double av = tile[(16 * rb + li) * LD + kk];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave has read the F tile
+
+    // ---- shift, interface sweep, assemble Z^T in the tile ----
+    double inv[RBK][JT][4];
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) {
+            const int j = 16 * (w * JT + jt) + li;
+            const double mu = j < n ? a.mu[j] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rb + 4 * r + lk;
+                const double d = mu - row_sig[row];
+                const double iv = (j < n) ? 1.0 / d : 0.0;
+                inv[rb][jt][r] = iv;
+                if (GENERAL) tile[row * LD + j] = acc[rb][jt][r] * iv;  // q
+            }
+        }
+    if (GENERAL) {
+        __syncthreads();
+        // end-point derivatives of the particular solutions: r0|rN . q
+        for (int item = tid; item < 2 * RT; item += 256) {
+            const int row = item >> 1, which = item & 1;
+            const double *rv = which ? a.rN : a.r0;
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s = fma(rv[j], tile[row * LD + j], s);
+            if (which) row_dN[row] = s;
+            else row_d0[row] = s;
+        }
+        __syncthreads();
+        // sweep over the elements of each real spectral column
+        for (int sc = tid; sc < 2 * fc_per_tile; sc += 256) {
+            const int fl = sc >> 1, plane = sc & 1;
+            const int fc = fc0 + fl;
+            if (fc >= nfc) continue;
+            const int i = fc / M, m = fc % M;
+            const int iw = i <= a.nx / 2 ? i : a.nx - i;
+            const int mode = iw * M + m;
+            const double *bsrc = plane ? a.bc_im : a.bc_re;
+            const double gl = bsrc[fc], gr = bsrc[(long long)nfc + fc];
+            const double d0ul = a.hom[4 * mode + 0], dNul = a.hom[4 * mode + 1];
+            const double d0ur = a.hom[4 * mode + 2], dNur = a.hom[4 * mode + 3];
+            const int r0w = (fl * 2 + plane) * E;
+            double bprev = -a.alpha_l * d0ul + a.beta_l;
+            double cprev = -a.alpha_l * d0ur;
+            double rprev = gl + a.alpha_l * row_d0[r0w];
+            row_zl[r0w] = bprev;  // b, rhs of the forward elimination kept in zl/zr
+            row_zr[r0w] = rprev;
+            for (int e = 1; e < E; ++e) {
+                const double lb = dNul;
+                const double be = dNur - d0ul - (cprev / bprev) * lb;
+                const double re = row_d0[r0w + e] - row_dN[r0w + e - 1] - (rprev / bprev) * lb;
+                bprev = be;
+                cprev = -d0ur;
+                rprev = re;
+                row_zl[r0w + e] = be;
+                row_zr[r0w + e] = re;
+            }
+            const double lb = a.alpha_r * dNul;
+            const double br = a.alpha_r * dNur + a.beta_r - (cprev / bprev) * lb;
+            const double rhsr = gr - a.alpha_r * row_dN[r0w + E - 1] - (rprev / bprev) * lb;
+            double zr = rhsr / br;
+            for (int e = E - 1; e >= 0; --e) {
+                const double be = row_zl[r0w + e], re = row_zr[r0w + e];
+                const double ce = (e == 0) ? -a.alpha_l * d0ur : -d0ur;
+                const double zl = (re - ce * zr) / be;
+                row_zl[r0w + e] = zl;
+                row_zr[r0w + e] = zr;
+                zr = zl;
+            }
+        }
+    } else {
+        // one element, Dirichlet data on both faces: zl = g_l, zr = g_r
+        for (int row = tid; row < RT; row += 256) {
+            const int fl = row >> 1, plane = row & 1;
+            const int fc = fc0 + fl;
+            double zl = 0.0, zr = 0.0;
+            if (fl < fc_per_tile && fc < nfc) {
+                const double *bsrc = plane ? a.bc_im : a.bc_re;
+                zl = bsrc[fc] / a.beta_l;
+                zr = bsrc[(long long)nfc + fc] / a.beta_r;
+            }
+            row_zl[row] = zl;
+            row_zr[row] = zr;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) {
+            const int j = 16 * (w * JT + jt) + li;
+            const double w0 = j < n ? a.w0[j] : 0.0, wN = j < n ? a.wN[j] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rb + 4 * r + lk;
+                const double h = fma(row_zl[row], w0, row_zr[row] * wN);
+                tile[row * LD + j] = (acc[rb][jt][r] + h) * inv[rb][jt][r];
+            }
+        }
+    __syncthreads();
+
+    // ---- U^T = Z^T V^T, stored along z ----
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < NP / 4; ++ks) {
+        const int kk = 4 * ks + lk;
+        double bv[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VT[(long long)kk * NP + 16 * (w * JT + jt) + li];
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb) {
+            const double av = tile[(16 * rb + li) * LD + kk];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) {
+            const int k = 16 * (w * JT + jt) + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rb + 4 * r + lk;
+                const long long col = row_col[row];
+                if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
+            }
+        }
+    // element end values
+    for (int row = tid; row < RT; row += 256) {
+        const long long col = row_col[row];
+        if (col >= 0) {
+            a.spec[col] = row_zl[row];
+            a.spec[col + N - 1] = row_zr[row];
+        }
+    }
+}
+
+template <int NJ, int RT>
+int launch_t(const DiagArgs &a, hipStream_t s)
+{
+    const int rows_per_fc = 2 * a.ne;
+    const int fc_per_tile = RT / rows_per_fc;
+    if (fc_per_tile < 1) return set_error(HZ_ERR_UNSUPPORTED, "DIAG form: too many elements for one tile");
+    const int nfc = a.nx * a.M;
+    const int blocks = (nfc + fc_per_tile - 1) / fc_per_tile;
+    const bool general = a.ne > 1 || a.alpha_l != 0.0 || a.alpha_r != 0.0;
+    if (general)
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true>), dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false>), dim3(blocks), dim3(256), 0, s, a);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+}  // namespace
+
+int diag_padded(int n)
+{
+    if (n <= 64) return 64;
+    if (n <= 128) return 128;
+    if (n <= 256) return 256;
+    return 0;
+}
+
+int launch_diag_modes(const DiagArgs &a, hipStream_t s)
+{
+    switch (diag_padded(a.n)) {
+    case 64: return launch_t<4, 64>(a, s);
+    case 128: return launch_t<8, 64>(a, s);
+    case 256: return launch_t<16, 32>(a, s);
+    default: return set_error(HZ_ERR_UNSUPPORTED, "DIAG form: N-2 must be <= 256");
+    }
+}
+
+}  // namespace hz

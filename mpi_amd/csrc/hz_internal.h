@@ -70,6 +70,21 @@ struct LuModeArgs {
 };
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s);
 
+// Arguments of the z-diagonalised mode kernel (diag.hip).
+struct DiagArgs {
+    int nx, M, K, N, n, ne;
+    double *spec;               // real plane; imaginary plane at spec + plane_off
+    long long plane_off;
+    const double *bc_re, *bc_im;
+    const double *shift;        // [modes]
+    const double *mu, *w0, *wN, *r0, *rN;  // [n]
+    const double *VinvT, *VT;   // [NP][NP], NP = diag_padded(n)
+    const double *hom;          // [modes][4]: Dz0.ul, DzN.ul, Dz0.ur, DzN.ur
+    double alpha_l, beta_l, alpha_r, beta_r;
+};
+int diag_padded(int n);
+int launch_diag_modes(const DiagArgs &a, hipStream_t s);
+
 struct LuFactorArgs {
     double *A;
     int n, count;
@@ -94,6 +109,11 @@ struct hz_plan {
     double2 *d_twx = nullptr, *d_twy = nullptr;
     hz::Spectrum spec, bc;
     int *d_status = nullptr;
+    // z-diagonalised form
+    int NP = 0;
+    std::vector<double> mu, V, Vinv, w0, wN, r0, rN;
+    double *d_mu = nullptr, *d_w0 = nullptr, *d_wN = nullptr, *d_r0 = nullptr, *d_rN = nullptr;
+    double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     // stage profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;      // events in use, per solve: nst+1

@@ -72,12 +72,39 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->d_shift);
     (void)hipFree(p->d_twx);
     (void)hipFree(p->d_twy);
-    (void)hipFree(p->spec.re);
-    (void)hipFree(p->spec.im);
+    (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
-    (void)hipFree(p->bc.im);
+    for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom})
+        (void)hipFree(q);
     (void)hipFree(p->d_status);
     delete p;
+}
+
+// Per-mode end-point derivatives of the homogeneous solutions in the
+// diagonalised basis: Dz[0|N-1].ul and Dz[0|N-1].ur for the mode's shift.
+int compute_homogeneous(hz_plan *p)
+{
+    if (!p->d_hom) return HZ_OK;
+    const int n = p->n, N = p->N;
+    std::vector<double> hom((size_t)4 * p->modes);
+    for (int m = 0; m < p->modes; ++m) {
+        long double a0 = p->Dz[0], aN = p->Dz[(size_t)(N - 1) * N];
+        long double b0 = p->Dz[N - 1], bN = p->Dz[(size_t)(N - 1) * N + N - 1];
+        const long double sig = p->shift[m];
+        for (int j = 0; j < n; ++j) {
+            const long double iv = 1.0L / ((long double)p->mu[j] - sig);
+            a0 += (long double)p->r0[j] * p->w0[j] * iv;
+            aN += (long double)p->rN[j] * p->w0[j] * iv;
+            b0 += (long double)p->r0[j] * p->wN[j] * iv;
+            bN += (long double)p->rN[j] * p->wN[j] * iv;
+        }
+        hom[4 * m + 0] = (double)a0;
+        hom[4 * m + 1] = (double)aN;
+        hom[4 * m + 2] = (double)b0;
+        hom[4 * m + 3] = (double)bN;
+    }
+    HZ_HIP(hipMemcpy(p->d_hom, hom.data(), sizeof(double) * hom.size(), hipMemcpyHostToDevice));
+    return HZ_OK;
 }
 
 int compute_shifts(hz_plan *p)
@@ -87,6 +114,49 @@ int compute_shifts(hz_plan *p)
         for (int jw = 0; jw < p->M; ++jw)
             p->shift[(size_t)iw * p->M + jw] = mode_shift(p->d.lambda, iw, jw, p->Lx, p->Ly);
     HZ_HIP(hipMemcpy(p->d_shift, p->shift.data(), sizeof(double) * p->modes, hipMemcpyHostToDevice));
+    return compute_homogeneous(p);
+}
+
+// Eigen-decomposition of the interior operator and the derived vectors of
+// the diagonalised form, uploaded padded to NP x NP.
+int setup_diag(hz_plan *p)
+{
+    const int n = p->n, N = p->N;
+    p->NP = diag_padded(n);
+    if (!p->NP) return set_error(HZ_ERR_UNSUPPORTED, "DIAG form: N-2 must be <= 256");
+    std::vector<double> A((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = p->Dzz[(size_t)(i + 1) * N + (j + 1)];
+    int st = eigen_decompose(A, n, p->mu, p->V, p->Vinv);
+    if (st) return set_error(st, "eigendecomposition of the GLL operator did not converge");
+    p->w0.assign(n, 0.0);
+    p->wN.assign(n, 0.0);
+    p->r0.assign(n, 0.0);
+    p->rN.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        long double s0 = 0, sN = 0, t0 = 0, tN = 0;
+        for (int k = 0; k < n; ++k) {
+            s0 += (long double)p->Vinv[(size_t)i * n + k] * -p->Dzz[(size_t)(k + 1) * N];
+            sN += (long double)p->Vinv[(size_t)i * n + k] * -p->Dzz[(size_t)(k + 1) * N + N - 1];
+            t0 += (long double)p->Dz[k + 1] * p->V[(size_t)k * n + i];
+            tN += (long double)p->Dz[(size_t)(N - 1) * N + k + 1] * p->V[(size_t)k * n + i];
+        }
+        p->w0[i] = (double)s0;
+        p->wN[i] = (double)sN;
+        p->r0[i] = (double)t0;
+        p->rN[i] = (double)tN;
+    }
+    const int NP = p->NP;
+    std::vector<double> VinvT((size_t)NP * NP, 0.0), VT((size_t)NP * NP, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            VinvT[(size_t)j * NP + i] = p->Vinv[(size_t)i * n + j];
+            VT[(size_t)j * NP + i] = p->V[(size_t)i * n + j];
+        }
+    if ((st = upload(&p->d_mu, p->mu)) || (st = upload(&p->d_w0, p->w0)) || (st = upload(&p->d_wN, p->wN)) ||
+        (st = upload(&p->d_r0, p->r0)) || (st = upload(&p->d_rN, p->rN)) || (st = upload(&p->d_VinvT, VinvT)) ||
+        (st = upload(&p->d_VT, VT)) || (st = dmalloc(&p->d_hom, (size_t)4 * p->modes)))
+        return st;
     return HZ_OK;
 }
 
@@ -241,12 +311,14 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     if ((st = upload(&p->d_Dz, p->Dz)) || (st = upload(&p->d_Dzz, p->Dzz)) ||
         (st = upload(&p->d_DzzT, DzzT)) || (st = dmalloc(&p->d_shift, p->modes)) ||
         (st = upload(&p->d_twx, twiddles(p->nx))) || (st = upload(&p->d_twy, twiddles(p->ny))) ||
-        (st = dmalloc(&p->spec.re, specN)) || (st = dmalloc(&p->spec.im, specN)) ||
-        (st = dmalloc(&p->bc.re, bcN)) || (st = dmalloc(&p->bc.im, bcN)) ||
-        (st = dmalloc(&p->d_status, p->modes)) || (st = compute_shifts(p))) {
+        (st = dmalloc(&p->spec.re, 2 * specN)) || (st = dmalloc(&p->bc.re, 2 * bcN)) ||
+        (st = dmalloc(&p->d_status, p->modes)) ||
+        (d.form == HZ_FORM_DIAG && (st = setup_diag(p))) || (st = compute_shifts(p))) {
         free_plan(p);
         return st;
     }
+    p->spec.im = p->spec.re + specN;
+    p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
     *plan = p;
     return HZ_OK;
@@ -330,6 +402,33 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.status = p->d_status;
         st = launch_lu_modes(a, p->modes, s);
         rec.mark("lu_modes");
+    } else if (p->d.form == HZ_FORM_DIAG) {
+        DiagArgs a{};
+        a.nx = p->nx;
+        a.M = p->M;
+        a.K = p->K;
+        a.N = p->N;
+        a.n = p->n;
+        a.ne = p->ne;
+        a.spec = p->spec.re;
+        a.plane_off = p->spec.im - p->spec.re;
+        a.bc_re = p->bc.re;
+        a.bc_im = p->bc.im;
+        a.shift = p->d_shift;
+        a.mu = p->d_mu;
+        a.w0 = p->d_w0;
+        a.wN = p->d_wN;
+        a.r0 = p->d_r0;
+        a.rN = p->d_rN;
+        a.VinvT = p->d_VinvT;
+        a.VT = p->d_VT;
+        a.hom = p->d_hom;
+        a.alpha_l = p->d.alpha_l;
+        a.beta_l = p->d.beta_l;
+        a.alpha_r = p->d.alpha_r;
+        a.beta_r = p->d.beta_r;
+        st = launch_diag_modes(a, s);
+        rec.mark("diag_modes");
     } else {
         st = set_error(HZ_ERR_UNSUPPORTED, "form not built yet");
     }
