@@ -171,6 +171,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--forms", default="plain,diag")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (read + write), the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -178,6 +181,12 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib * (1 << 17), dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # reads and writes exactly copy_probe_mib MiB each
+        torch.cuda.synchronize()
+        del src, dst
     from mpi_amd.solver import HelmholtzPlan
     probe = HelmholtzPlan(NX, NY, N, 1, lam=LAM, form="plain")
     f, gl, gr = make_inputs(probe)

@@ -70,44 +70,61 @@ __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArg
 {
     static_assert(R1 * R2 == L, "radix split");
     constexpr int SIGN = (KIND == FFT_R2C || KIND == FFT_C2C_FWD) ? -1 : 1;
-    __shared__ double tre[L * W], tim[L * W];
+    // One LDS tile, used twice: real parts, then imaginary parts.
+    __shared__ double tile[L * W];
 
     const int w = threadIdx.x % W, q = threadIdx.x / W;
     const int lane = blockIdx.x * W + w;
     const int batch = blockIdx.y;
-    const bool live = lane < a.lanes;
     const double2 *__restrict__ tw = a.tw;
+    // lane source: the main field, or one of the extra single-lane fields
+    const double *in_re = a.in_re, *in_im = a.in_im;
+    double *out_re = a.out_re, *out_im = a.out_im;
+    long long ibase = (long long)batch * a.in_batch_stride + lane, ips = a.in_point_stride;
+    long long obase = (long long)batch * a.out_batch_stride + lane, ops = a.out_point_stride;
+    bool live = lane < a.lanes;
+    if (!live && lane < a.lanes + a.xlanes) {
+        const int sx = lane - a.lanes;
+        in_re = a.xin_re[sx];
+        in_im = a.xin_im[sx];
+        out_re = a.xout_re[sx];
+        out_im = a.xout_im[sx];
+        ibase = (long long)batch * a.x_in_batch_stride;
+        ips = a.x_in_point_stride;
+        obase = (long long)batch * a.x_out_batch_stride;
+        ops = a.x_out_point_stride;
+        live = true;
+    }
 
+    double sre[R1], sim[R1];
     if (q < R2) {
         const int n2 = q;
-        double re[R1], im[R1];
-        const long long base = (long long)batch * a.in_batch_stride + lane;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
             const int p = n2 + R2 * n1;
             double vr = 0.0, vi = 0.0;
             if (live) {
                 if (KIND == FFT_R2C) {
-                    vr = a.in_re[base + (long long)p * a.in_point_stride];
+                    vr = in_re[ibase + (long long)p * ips];
                 } else if (KIND == FFT_C2R) {
                     const int pp = p <= L / 2 ? p : L - p;
-                    const long long off = base + (long long)pp * a.in_point_stride;
-                    vr = a.in_re[off];
-                    vi = a.in_im[off];
+                    const long long off = ibase + (long long)pp * ips;
+                    vr = in_re[off];
+                    vi = in_im[off];
                     if (p > L / 2) vi = -vi;
                 } else {
-                    const long long off = base + (long long)p * a.in_point_stride;
-                    vr = a.in_re[off];
-                    vi = a.in_im[off];
+                    const long long off = ibase + (long long)p * ips;
+                    vr = in_re[off];
+                    vi = in_im[off];
                 }
             }
-            re[n1] = vr;
-            im[n1] = vi;
+            sre[n1] = vr;
+            sim[n1] = vi;
         }
-        dft_regs<R1, L, SIGN>(re, im, tw);
+        dft_regs<R1, L, SIGN>(sre, sim, tw);
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
-            double vr = re[k1], vi = im[k1];
+            double vr = sre[k1], vi = sim[k1];
             const int t = (n2 * k1) % L;
             if (t != 0) {
                 const double2 tt = tw[t];
@@ -116,40 +133,48 @@ __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArg
                 vi = vr * wi + vi * wr;
                 vr = xr;
             }
-            tre[(k1 * R2 + n2) * W + w] = vr;
-            tim[(k1 * R2 + n2) * W + w] = vi;
+            sre[k1] = vr;
+            sim[k1] = vi;
+            tile[(k1 * R2 + n2) * W + w] = vr;
         }
+    }
+    __syncthreads();
+    double re[R2], im[R2];
+    if (q < R1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) re[n2] = tile[(q * R2 + n2) * W + w];
+    }
+    __syncthreads();
+    if (q < R2) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) tile[(k1 * R2 + q) * W + w] = sim[k1];
     }
     __syncthreads();
     if (q < R1) {
         const int k1 = q;
-        double re[R2], im[R2];
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) {
-            re[n2] = tre[(k1 * R2 + n2) * W + w];
-            im[n2] = tim[(k1 * R2 + n2) * W + w];
-        }
+        for (int n2 = 0; n2 < R2; ++n2) im[n2] = tile[(k1 * R2 + n2) * W + w];
         dft_regs<R2, L, SIGN>(re, im, tw);
         if (live) {
-            const long long base = (long long)batch * a.out_batch_stride + lane;
+            const long long base = obase;
             constexpr double scale = 1.0 / L;
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) {
                 const int kk = k1 + R1 * k2;
-                const long long off = base + (long long)kk * a.out_point_stride;
+                const long long off = base + (long long)kk * ops;
                 if (KIND == FFT_R2C) {
                     if (kk <= L / 2) {
-                        a.out_re[off] = re[k2];
-                        a.out_im[off] = im[k2];
+                        out_re[off] = re[k2];
+                        out_im[off] = im[k2];
                     }
                 } else if (KIND == FFT_C2R) {
-                    a.out_re[off] = re[k2] * scale;
+                    out_re[off] = re[k2] * scale;
                 } else if (KIND == FFT_C2C_INV) {
-                    a.out_re[off] = re[k2] * scale;
-                    a.out_im[off] = im[k2] * scale;
+                    out_re[off] = re[k2] * scale;
+                    out_im[off] = im[k2] * scale;
                 } else {
-                    a.out_re[off] = re[k2];
-                    a.out_im[off] = im[k2];
+                    out_re[off] = re[k2];
+                    out_im[off] = im[k2];
                 }
             }
         }
@@ -160,7 +185,7 @@ template <int L, int R1, int R2, int W>
 int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
 {
     constexpr int Q = R1 > R2 ? R1 : R2;
-    dim3 grid((a.lanes + W - 1) / W, a.nbatch), block(W * Q);
+    dim3 grid((a.lanes + a.xlanes + W - 1) / W, a.nbatch), block(W * Q);
     switch (kind) {
     case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
     case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
@@ -175,7 +200,7 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
 
 int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
 {
-    if (a.nbatch <= 0 || a.lanes <= 0) return HZ_OK;
+    if (a.nbatch <= 0 || a.lanes + a.xlanes <= 0) return HZ_OK;
     switch (L) {
     case 2: return launch_kind<2, 2, 1, 16>(kind, a, s);
     case 4: return launch_kind<4, 4, 1, 16>(kind, a, s);

@@ -51,6 +51,13 @@ struct FftArgs {
     int lanes;                     // contiguous lanes (K)
     int nbatch;
     const double2 *tw;             // tw[t] = (cos 2 pi t/L, sin 2 pi t/L)
+    // Up to two extra single-lane fields transformed in the same launch
+    // (the boundary faces): lane `lanes + s` reads xin_*[s] and writes
+    // xout_*[s] with the x_* strides.
+    int xlanes;
+    const double *xin_re[2], *xin_im[2];
+    double *xout_re[2], *xout_im[2];
+    long long x_in_batch_stride, x_in_point_stride, x_out_batch_stride, x_out_point_stride;
 };
 int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s);
 

@@ -160,10 +160,27 @@ int setup_diag(hz_plan *p)
     return HZ_OK;
 }
 
-// Forward x-y transform of a [nx][ny][lanes] field into split half spectra.
-int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s)
+// Forward x-y transform of a [nx][ny][lanes] field into split half spectra;
+// with gl/gr the two face planes ride along as extra lanes of the same
+// launches and land in the boundary spectra.
+int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s,
+               const double *gl = nullptr, const double *gr = nullptr)
 {
+    const long long side = (long long)p->nx * p->M;
     FftArgs a{};
+    if (gl && gr) {
+        a.xlanes = 2;
+        a.xin_re[0] = gl;
+        a.xin_re[1] = gr;
+        a.xout_re[0] = p->bc.re;
+        a.xout_re[1] = p->bc.re + side;
+        a.xout_im[0] = p->bc.im;
+        a.xout_im[1] = p->bc.im + side;
+        a.x_in_batch_stride = p->ny;
+        a.x_in_point_stride = 1;
+        a.x_out_batch_stride = p->M;
+        a.x_out_point_stride = 1;
+    }
     a.in_re = in;
     a.out_re = re;
     a.out_im = im;
@@ -188,6 +205,15 @@ int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, 
     b.lanes = lanes;
     b.nbatch = p->M;
     b.tw = p->d_twx;
+    if (gl && gr) {
+        b.xlanes = 2;
+        for (int sx = 0; sx < 2; ++sx) {
+            b.xin_re[sx] = b.xout_re[sx] = p->bc.re + sx * side;
+            b.xin_im[sx] = b.xout_im[sx] = p->bc.im + sx * side;
+        }
+        b.x_in_batch_stride = b.x_out_batch_stride = 1;
+        b.x_in_point_stride = b.x_out_point_stride = p->M;
+    }
     return launch_fft(p->nx, FFT_C2C_FWD, b, s);
 }
 
@@ -221,13 +247,6 @@ int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipSt
     return launch_fft(p->ny, FFT_C2R, a, s);
 }
 
-int forward_bc(hz_plan *p, const double *gl, const double *gr, hipStream_t s)
-{
-    const size_t side = (size_t)p->nx * p->M;
-    int st = forward_xy(p, gl, p->bc.re, p->bc.im, 1, s);
-    if (st) return st;
-    return forward_xy(p, gr, p->bc.re + side, p->bc.im + side, 1, s);
-}
 
 // Records stage boundaries of one solve when profiling is on.
 struct StageRecorder {
@@ -371,10 +390,8 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
 {
     if (!p || !f || !gl || !gr || !u) return set_error(HZ_ERR_ARG, "null argument");
     StageRecorder rec(p, s);
-    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s);
+    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr);
     rec.mark("fft_forward");
-    if (!st) st = forward_bc(p, gl, gr, s);
-    rec.mark("fft_faces");
     if (st) return st;
     if (p->d.form == HZ_FORM_PLAIN_LU || op_diag || op_full) {
         LuModeArgs a{};
