@@ -95,6 +95,8 @@ int hz_plan_operators(const hz_plan *plan, double *Dz, double *Dzz);
 int hz_plan_num_modes(const hz_plan *plan, int *num_modes); /* (nx/2+1)(ny/2+1) */
 /* Changes lambda (and with it every operator) for later solves. */
 int hz_plan_set_lambda(hz_plan *plan, double lambda);
+/* Changes the Robin coefficients of the two faces for later solves. */
+int hz_plan_set_robin(hz_plan *plan, double alpha_l, double beta_l, double alpha_r, double beta_r);
 
 /* Replaces SolvePDE (NonPeriodic3D_MPI.cpp:237-591) for the standard
  * operator of the reference test callback (TestNonPeriodic3D_MPI.cpp:71-79):
@@ -129,8 +131,10 @@ int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
                double *glbar, double *grbar, double *lambda_bar, hz_stream_t stream);
 
 /* Replaces FirstDerivativeZ / SecondDerivativeZ (NonPeriodic3D_MPI.cpp:632-683):
- * out = Dz u (order 1) or Dzz u (order 2) element by element; at an element
- * interface the later element's value is stored. */
+ * out = Dz u (order 1) or Dzz u (order 2) applied to every element column.
+ * Here u and out use the element layout of the reference's FieldVariable
+ * arrays, [ne][nx][ny][n] (z fastest, interface nodes stored in both
+ * neighbouring elements), device pointers. */
 int hz_derivative_z(hz_plan *plan, int order, const double *u, double *out,
                     hz_stream_t stream);
 

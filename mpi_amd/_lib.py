@@ -46,6 +46,7 @@ SIGNATURES = {
     "hz_plan_operators": (ctypes.c_int, [_VP, _PD, _PD]),
     "hz_plan_num_modes": (ctypes.c_int, [_VP, _PI]),
     "hz_plan_set_lambda": (ctypes.c_int, [_VP, _D]),
+    "hz_plan_set_robin": (ctypes.c_int, [_VP, _D, _D, _D, _D]),
     "hz_solve": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "hz_solve_operators": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "hz_solve_status": (ctypes.c_int, [_VP, _VP]),

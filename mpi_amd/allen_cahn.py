@@ -1,0 +1,117 @@
+"""Semi-implicit Allen-Cahn stepping on the Helmholtz solver, differentiated
+with the adjoint kernels and step checkpointing.
+
+    u_t = eps^2 lap(u) + u - u^3      on [0, 2 pi)^2 x [-1, 1]
+
+Each step is one Helmholtz solve (z faces homogeneous Neumann:
+alpha = 1, beta = 0, g = 0):
+
+    lap(u1) - lam u1 = -lam u0 + (u0^3 - u0) / eps^2,   lam = 1 / (dt eps^2)
+
+The loss is the quadrature  L = sum_k W_k (u_T - u_target)_k^2  with
+uniform weights in x, y and Gauss-Lobatto-Legendre weights in z.
+
+``loss_and_grad`` returns L, dL/du_0 and dL/deps.  It runs the forward pass
+keeping a checkpoint every ``checkpoint_every`` steps, then walks the
+segments backwards: each segment is recomputed from its checkpoint and the
+adjoint of each step is applied with hz_adjoint (through the plan).  With
+ubar = dL/du1 for the step u0 -> u1 and (fbar, lam_bar) = S^T ubar:
+
+    dL/du0  = fbar * (-lam + (3 u0^2 - 1) / eps^2)
+    dL/deps += lam_bar dlam/deps + <fbar, -u0 dlam/deps - 2 (u0^3 - u0) / eps^3>
+    dlam/deps = -2 / (dt eps^3)
+
+Everything stays on the device; the only host transfer per step is the
+scalar lam_bar.  The reference has no time stepper; this module is the
+differentiation workload built on the solver.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .solver import HelmholtzPlan
+
+
+def gll_weights(x):
+    """Quadrature weights of GLL nodes x[0..N-1] on [-1, 1]:
+    w_i = 2 / (N (N-1) P_{N-1}(x_i)^2)."""
+    N = len(x)
+    p0 = np.ones_like(x)
+    p1 = np.array(x, dtype=np.float64)
+    for k in range(2, N):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    return 2.0 / (N * (N - 1) * p1 * p1)
+
+
+class AllenCahn:
+    def __init__(self, nx=128, ny=128, n=65, eps=0.02, dt=1e-3, steps=64, checkpoint_every=8,
+                 form="diag", Lx=2 * math.pi, Ly=2 * math.pi, zl=-1.0, zr=1.0):
+        self.nx, self.ny, self.n = nx, ny, n
+        self.dt, self.steps, self.every = dt, steps, checkpoint_every
+        self.eps0 = eps
+        self.plan = HelmholtzPlan(nx, ny, n, 1, xl=0.0, xr=Lx, yl=0.0, yr=Ly, zl=zl, zr=zr,
+                                  lam=self.lam_of(eps), alpha_l=1.0, beta_l=0.0, alpha_r=1.0, beta_r=0.0,
+                                  form=form)
+        self.K = self.plan.nz
+        z = self.plan.z_nodes()
+        xi = 2.0 * (z - z[0]) / (z[-1] - z[0]) - 1.0
+        wz = gll_weights(xi) * (zr - zl) / 2.0
+        self.W = torch.from_numpy(np.broadcast_to(wz * (Lx / nx) * (Ly / ny), (nx, ny, self.K)).copy()).cuda()
+        self.g0 = torch.zeros((nx, ny), dtype=torch.float64, device="cuda")
+
+    def lam_of(self, eps):
+        return 1.0 / (self.dt * eps * eps)
+
+    def _set_eps(self, eps):
+        lam = self.lam_of(eps)
+        if self.plan.desc.lam != lam:
+            self.plan.set_lambda(lam)
+        return lam
+
+    def step(self, u, eps, lam, out=None):
+        f = (u * u * u - u) * (1.0 / (eps * eps)) - lam * u
+        return self.plan.solve(f, self.g0, self.g0, out=out)
+
+    def run(self, u0, eps):
+        """u_T after `steps` steps (no checkpoints kept)."""
+        lam = self._set_eps(eps)
+        u = u0.contiguous()
+        for _ in range(self.steps):
+            u = self.step(u, eps, lam)
+        return u
+
+    def loss(self, uT, target):
+        d = uT - target
+        return float(torch.sum(self.W * d * d))
+
+    def loss_and_grad(self, u0, eps, target):
+        lam = self._set_eps(eps)
+        dlam = -2.0 / (self.dt * eps ** 3)
+        # forward with checkpoints
+        ckpt = {}
+        u = u0.contiguous()
+        for t in range(self.steps):
+            if t % self.every == 0:
+                ckpt[t] = u
+            u = self.step(u, eps, lam)
+        uT = u
+        L = self.loss(uT, target)
+        ubar = 2.0 * self.W * (uT - target)
+        eps_bar = 0.0
+        for s in sorted(ckpt, reverse=True):
+            e = min(s + self.every, self.steps)
+            states = [ckpt[s]]
+            for _ in range(s, e):
+                states.append(self.step(states[-1], eps, lam))
+            for t in range(e - 1, s - 1, -1):
+                u_in, u_out = states[t - s], states[t - s + 1]
+                fbar, _, _, lam_bar = self.plan.adjoint(ubar, u_out)
+                inv_e2 = 1.0 / (eps * eps)
+                eps_bar += lam_bar * dlam
+                eps_bar += float(torch.sum(fbar * (-dlam * u_in - (2.0 / eps ** 3) * (u_in * u_in * u_in - u_in))))
+                ubar = fbar * (-lam + (3.0 * u_in * u_in - 1.0) * inv_e2)
+        return L, ubar, eps_bar
+
+    def close(self):
+        self.plan.close()

@@ -15,6 +15,15 @@
 //   u_e = V ((y + zl_e w0 + zr_e wN) * inv)          (MFMA contraction)
 //   with u at the element ends equal to zl_e, zr_e (:582-587).
 //
+// Adjoint (ADJ = true, one element): the transposed column map.  With
+// ubar the column of dL/du, y = V^T ubar_int, inv as above,
+//   zbl = ubar_0 + (w0*inv).y,  zbr = ubar_{N-1} + (wN*inv).y
+// (the adjoints of zl, zr through u = p + zl ul + zr ur), the scalar sweep
+// run backwards gives dL/dg_l, dL/dg_r and the adjoints P0b, PNb of the
+// end-point derivatives, and
+//   fbar_int = V^{-T} ((y + P0b r0 + PNb rN) * inv),  fbar = 0 at the ends.
+// The kernel is the forward one with B1 = V, B2 = V^{-1}.
+//
 // A work-group owns RT tile rows (spectral column x element) and all NP
 // (padded n) eigen/grid indices.  Contractions are transposed so that MFMA
 // output lanes run along z, which is contiguous in memory:
@@ -33,7 +42,7 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <int NJ, int RT, bool GENERAL>
+template <int NJ, int RT, bool GENERAL, bool ADJ>
 __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
 {
     constexpr int NP = 16 * NJ;      // padded interior size
@@ -43,6 +52,7 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
     __shared__ double tile[RT * LD];
     __shared__ double row_sig[RT], row_zl[RT], row_zr[RT], row_d0[RT], row_dN[RT];
     __shared__ long long row_col[RT];  // global offset of the row's column start (element start)
+    __shared__ double row_u0[ADJ ? RT : 1], row_uN[ADJ ? RT : 1];
 
     const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
@@ -66,6 +76,10 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
         }
         row_sig[row] = sig;
         row_col[row] = col;
+        if constexpr (ADJ) {
+            row_u0[row] = col >= 0 ? a.spec[col] : 0.0;
+            row_uN[row] = col >= 0 ? a.spec[col + N - 1] : 0.0;
+        }
     }
     __syncthreads();
     // ---- stage F^T: tile[row][k] = f interior of the row's element ----
@@ -114,10 +128,49 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
                 const double d = mu - row_sig[row];
                 const double iv = (j < n) ? 1.0 / d : 0.0;
                 inv[rb][jt][r] = iv;
-                if (GENERAL) tile[row * LD + j] = acc[rb][jt][r] * iv;  // q
+                if (GENERAL || ADJ) tile[row * LD + j] = acc[rb][jt][r] * iv;  // q
             }
         }
-    if (GENERAL) {
+    if constexpr (ADJ) {
+        __syncthreads();
+        // (w0|wN * inv) . y: the adjoint load on zl, zr from the interior
+        for (int item = tid; item < 2 * RT; item += 256) {
+            const int row = item >> 1, which = item & 1;
+            const double *wv = which ? a.wN : a.w0;
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s = fma(wv[j], tile[row * LD + j], s);
+            if (which) row_dN[row] = s;
+            else row_d0[row] = s;
+        }
+        __syncthreads();
+        // transposed sweep (one element): zbar -> gbar, P0bar, PNbar
+        for (int row = tid; row < RT; row += 256) {
+            const int fl = row >> 1, plane = row & 1;
+            const int fc = fc0 + fl;
+            double P0b = 0.0, PNb = 0.0;
+            if (fl < fc_per_tile && fc < nfc) {
+                const int i = fc / M, m = fc % M;
+                const int iw = i <= a.nx / 2 ? i : a.nx - i;
+                const int mode = iw * M + m;
+                const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
+                const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
+                const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
+                const double lb = a.alpha_r * LN;
+                const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
+                const double zlb = row_u0[row] + row_d0[row];
+                const double zrb = row_uN[row] + row_dN[row] - (c / b) * zlb;
+                const double rhsrb = zrb / br;
+                const double rhsb = zlb / b - (lb / b) * rhsrb;
+                P0b = a.alpha_l * rhsb;
+                PNb = -a.alpha_r * rhsrb;
+                double *gre = plane ? a.gbar_im : a.gbar_re;
+                gre[fc] = rhsb;
+                gre[(long long)nfc + fc] = rhsrb;
+            }
+            row_zl[row] = P0b;  // reused: P0bar, PNbar per row
+            row_zr[row] = PNb;
+        }
+    } else if (GENERAL) {
         __syncthreads();
         // end-point derivatives of the particular solutions: r0|rN . q
         for (int item = tid; item < 2 * RT; item += 256) {
@@ -170,7 +223,7 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
                 zr = zl;
             }
         }
-    } else {
+    } else if constexpr (!ADJ) {
         // one element, Dirichlet data on both faces: zl = g_l, zr = g_r
         for (int row = tid; row < RT; row += 256) {
             const int fl = row >> 1, plane = row & 1;
@@ -186,12 +239,13 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
         }
     }
     __syncthreads();
+    const double *zv0 = ADJ ? a.r0 : a.w0, *zvN = ADJ ? a.rN : a.wN;
 #pragma unroll
     for (int rb = 0; rb < RBK; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (w * JT + jt) + li;
-            const double w0 = j < n ? a.w0[j] : 0.0, wN = j < n ? a.wN[j] : 0.0;
+            const double w0 = j < n ? zv0[j] : 0.0, wN = j < n ? zvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * rb + 4 * r + lk;
@@ -231,12 +285,12 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
                 if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
             }
         }
-    // element end values
+    // element end values (the adjoint load has none: f is not used there)
     for (int row = tid; row < RT; row += 256) {
         const long long col = row_col[row];
         if (col >= 0) {
-            a.spec[col] = row_zl[row];
-            a.spec[col + N - 1] = row_zr[row];
+            a.spec[col] = ADJ ? 0.0 : row_zl[row];
+            a.spec[col + N - 1] = ADJ ? 0.0 : row_zr[row];
         }
     }
 }
@@ -250,10 +304,14 @@ int launch_t(const DiagArgs &a, hipStream_t s)
     const int nfc = a.nx * a.M;
     const int blocks = (nfc + fc_per_tile - 1) / fc_per_tile;
     const bool general = a.ne > 1 || a.alpha_l != 0.0 || a.alpha_r != 0.0;
-    if (general)
-        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true>), dim3(blocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false>), dim3(blocks), dim3(256), 0, s, a);
+    if (a.adjoint) {
+        if (a.ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "adjoint: one element only");
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true>), dim3(blocks), dim3(256), 0, s, a);
+    } else if (general) {
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true, false>), dim3(blocks), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, false>), dim3(blocks), dim3(256), 0, s, a);
+    }
     HZ_HIP(hipGetLastError());
     return HZ_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -88,6 +89,8 @@ struct DiagArgs {
     const double *VinvT, *VT;   // [NP][NP], NP = diag_padded(n)
     const double *hom;          // [modes][4]: Dz0.ul, DzN.ul, Dz0.ur, DzN.ur
     double alpha_l, beta_l, alpha_r, beta_r;
+    int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
+    double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
 };
 int diag_padded(int n);
 int launch_diag_modes(const DiagArgs &a, hipStream_t s);
@@ -101,6 +104,9 @@ struct LuFactorArgs {
 };
 int launch_lu_factor(const LuFactorArgs &a, hipStream_t s);
 int launch_lu_solve(const double *LU, const int *piv, double *b, int n, int count, hipStream_t s);
+int launch_derivative(const double *D, int N, long long ncols, const double *u, double *out, hipStream_t s);
+// sum_i a[i]*b[i] into *out (device), deterministic two-pass reduction; partial >= 1024 doubles
+int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
 
 }  // namespace hz
 
@@ -121,6 +127,8 @@ struct hz_plan {
     std::vector<double> mu, V, Vinv, w0, wN, r0, rN;
     double *d_mu = nullptr, *d_w0 = nullptr, *d_wN = nullptr, *d_r0 = nullptr, *d_rN = nullptr;
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
+    double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
+    double *d_partial = nullptr;                      // reduction scratch
     // stage profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;      // events in use, per solve: nst+1

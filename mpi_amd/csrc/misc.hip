@@ -1,0 +1,88 @@
+// misc.hip -- element-wise GLL derivatives (FirstDerivativeZ /
+// SecondDerivativeZ, NonPeriodic3D_MPI.cpp:632-683).
+//
+// out[e][i][j][k] = sum_p D[k][p] u[e][i][j][p] for every element column:
+// a batch of small dense products, each thread producing one output node of
+// one column (columns are contiguous, so a wave reads and writes consecutive
+// z values; the N x N matrix stays in the caches).
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+__global__ __launch_bounds__(256) void derivative_kernel(const double *__restrict__ D, int N, long long ncols,
+                                                         const double *__restrict__ u, double *__restrict__ out)
+{
+    const long long total = ncols * N;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long col = idx / N;
+        const int k = (int)(idx % N);
+        const double *uc = u + col * N;
+        double acc = 0.0;
+        for (int p = 0; p < N; ++p) acc = fma(D[k * N + p], uc[p], acc);
+        out[idx] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                                          long long n, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc = fma(a[i], b[i], acc);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void dot_final_kernel(const double *__restrict__ partial, int np, double *out)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+
+int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s)
+{
+    const int blocks = (int)std::min<long long>(1024, std::max<long long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(256), 0, s, a, b, n, partial);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, partial, blocks, out);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+int launch_derivative(const double *D, int N, long long ncols, const double *u, double *out, hipStream_t s)
+{
+    if (ncols <= 0) return HZ_OK;
+    const long long total = ncols * N;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(derivative_kernel, dim3(blocks), dim3(256), 0, s, D, N, ncols, u, out);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+}  // namespace hz
+
+using namespace hz;
+
+extern "C" int hz_derivative_z(hz_plan *p, int order, const double *u, double *out, hz_stream_t stream)
+{
+    if (!p || !u || !out || (order != 1 && order != 2)) return set_error(HZ_ERR_ARG, "bad argument");
+    const long long ncols = (long long)p->ne * p->nx * p->ny;
+    return launch_derivative(order == 1 ? p->d_Dz : p->d_Dzz, p->N, ncols, u, out, (hipStream_t)stream);
+}

@@ -5,15 +5,6 @@
 using namespace hz;
 
 extern "C" {
-int hz_adjoint(hz_plan *, const double *, const double *, double *, double *, double *, double *,
-               hz_stream_t)
-{
-    return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: not built yet");
-}
-int hz_derivative_z(hz_plan *, int, const double *, double *, hz_stream_t)
-{
-    return set_error(HZ_ERR_UNSUPPORTED, "hz_derivative_z: not built yet");
-}
 int hz_batch_create(hz_plan *const *, int, hz_batch **)
 {
     return set_error(HZ_ERR_UNSUPPORTED, "hz_batch_create: not built yet");

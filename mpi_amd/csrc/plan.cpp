@@ -74,7 +74,8 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->d_twy);
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
-    for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom})
+    for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
+                      p->d_Vinvrow, p->d_partial})
         (void)hipFree(q);
     (void)hipFree(p->d_status);
     delete p;
@@ -148,14 +149,18 @@ int setup_diag(hz_plan *p)
     }
     const int NP = p->NP;
     std::vector<double> VinvT((size_t)NP * NP, 0.0), VT((size_t)NP * NP, 0.0);
+    std::vector<double> Vrow((size_t)NP * NP, 0.0), Vinvrow((size_t)NP * NP, 0.0);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             VinvT[(size_t)j * NP + i] = p->Vinv[(size_t)i * n + j];
             VT[(size_t)j * NP + i] = p->V[(size_t)i * n + j];
+            Vrow[(size_t)i * NP + j] = p->V[(size_t)i * n + j];
+            Vinvrow[(size_t)i * NP + j] = p->Vinv[(size_t)i * n + j];
         }
     if ((st = upload(&p->d_mu, p->mu)) || (st = upload(&p->d_w0, p->w0)) || (st = upload(&p->d_wN, p->wN)) ||
         (st = upload(&p->d_r0, p->r0)) || (st = upload(&p->d_rN, p->rN)) || (st = upload(&p->d_VinvT, VinvT)) ||
-        (st = upload(&p->d_VT, VT)) || (st = dmalloc(&p->d_hom, (size_t)4 * p->modes)))
+        (st = upload(&p->d_VT, VT)) || (st = upload(&p->d_Vrow, Vrow)) || (st = upload(&p->d_Vinvrow, Vinvrow)) ||
+        (st = dmalloc(&p->d_hom, (size_t)4 * p->modes)))
         return st;
     return HZ_OK;
 }
@@ -217,9 +222,20 @@ int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, 
     return launch_fft(p->nx, FFT_C2C_FWD, b, s);
 }
 
-int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipStream_t s)
+int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipStream_t s,
+               double *gl_out = nullptr, double *gr_out = nullptr)
 {
+    const long long side = (long long)p->nx * p->M;
     FftArgs b{};
+    if (gl_out && gr_out) {
+        b.xlanes = 2;
+        for (int sx = 0; sx < 2; ++sx) {
+            b.xin_re[sx] = b.xout_re[sx] = p->bc.re + sx * side;
+            b.xin_im[sx] = b.xout_im[sx] = p->bc.im + sx * side;
+        }
+        b.x_in_batch_stride = b.x_out_batch_stride = 1;
+        b.x_in_point_stride = b.x_out_point_stride = p->M;
+    }
     b.in_re = re;
     b.in_im = im;
     b.out_re = re;
@@ -244,9 +260,51 @@ int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipSt
     a.lanes = lanes;
     a.nbatch = p->nx;
     a.tw = p->d_twy;
+    if (gl_out && gr_out) {
+        a.xlanes = 2;
+        for (int sx = 0; sx < 2; ++sx) {
+            a.xin_re[sx] = p->bc.re + sx * side;
+            a.xin_im[sx] = p->bc.im + sx * side;
+        }
+        a.xout_re[0] = gl_out;
+        a.xout_re[1] = gr_out;
+        a.x_in_batch_stride = p->M;
+        a.x_in_point_stride = 1;
+        a.x_out_batch_stride = p->ny;
+        a.x_out_point_stride = 1;
+    }
     return launch_fft(p->ny, FFT_C2R, a, s);
 }
 
+// Kernel arguments of the z-diagonalised mode solve (forward map).
+DiagArgs diag_args(hz_plan *p)
+{
+    DiagArgs a{};
+    a.nx = p->nx;
+    a.M = p->M;
+    a.K = p->K;
+    a.N = p->N;
+    a.n = p->n;
+    a.ne = p->ne;
+    a.spec = p->spec.re;
+    a.plane_off = p->spec.im - p->spec.re;
+    a.bc_re = p->bc.re;
+    a.bc_im = p->bc.im;
+    a.shift = p->d_shift;
+    a.mu = p->d_mu;
+    a.w0 = p->d_w0;
+    a.wN = p->d_wN;
+    a.r0 = p->d_r0;
+    a.rN = p->d_rN;
+    a.VinvT = p->d_VinvT;
+    a.VT = p->d_VT;
+    a.hom = p->d_hom;
+    a.alpha_l = p->d.alpha_l;
+    a.beta_l = p->d.beta_l;
+    a.alpha_r = p->d.alpha_r;
+    a.beta_r = p->d.beta_r;
+    return a;
+}
 
 // Records stage boundaries of one solve when profiling is on.
 struct StageRecorder {
@@ -385,6 +443,16 @@ int hz_plan_set_lambda(hz_plan *p, double lambda)
     return compute_shifts(p);
 }
 
+int hz_plan_set_robin(hz_plan *p, double alpha_l, double beta_l, double alpha_r, double beta_r)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    p->d.alpha_l = alpha_l;
+    p->d.beta_l = beta_l;
+    p->d.alpha_r = alpha_r;
+    p->d.beta_r = beta_r;
+    return HZ_OK;
+}
+
 static int solve_impl(hz_plan *p, const double *f, const double *gl, const double *gr,
                       const double *op_diag, const double *op_full, double *u, hipStream_t s)
 {
@@ -420,30 +488,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         st = launch_lu_modes(a, p->modes, s);
         rec.mark("lu_modes");
     } else if (p->d.form == HZ_FORM_DIAG) {
-        DiagArgs a{};
-        a.nx = p->nx;
-        a.M = p->M;
-        a.K = p->K;
-        a.N = p->N;
-        a.n = p->n;
-        a.ne = p->ne;
-        a.spec = p->spec.re;
-        a.plane_off = p->spec.im - p->spec.re;
-        a.bc_re = p->bc.re;
-        a.bc_im = p->bc.im;
-        a.shift = p->d_shift;
-        a.mu = p->d_mu;
-        a.w0 = p->d_w0;
-        a.wN = p->d_wN;
-        a.r0 = p->d_r0;
-        a.rN = p->d_rN;
-        a.VinvT = p->d_VinvT;
-        a.VT = p->d_VT;
-        a.hom = p->d_hom;
-        a.alpha_l = p->d.alpha_l;
-        a.beta_l = p->d.beta_l;
-        a.alpha_r = p->d.alpha_r;
-        a.beta_r = p->d.beta_r;
+        DiagArgs a = diag_args(p);
         st = launch_diag_modes(a, s);
         rec.mark("diag_modes");
     } else {
@@ -513,6 +558,47 @@ int hz_solve_status(hz_plan *p, hz_stream_t stream)
     HZ_HIP(hipMemcpy(st.data(), p->d_status, sizeof(int) * p->modes, hipMemcpyDeviceToHost));
     for (int m = 0; m < p->modes; ++m)
         if (st[m]) return set_error(HZ_ERR_SINGULAR, "ERROR: Matrix is singular!");
+    return HZ_OK;
+}
+
+// Adjoint of the solve map.  The x-y transform pair and the per-mode
+// operators commute with translations in x and y, so the transpose of
+// F^-1 M F is F^-1 M^T F: the same pipeline with the transposed column map
+// (diag.hip, ADJ) in the middle.  A PLAIN_LU plan builds the diagonalised
+// operators on its first adjoint call; both forms solve the same discrete
+// system, so they share one adjoint.
+int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar,
+               double *grbar, double *lambda_bar, hz_stream_t stream)
+{
+    if (!p || !ubar || !fbar || !glbar || !grbar) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: one element only");
+    if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
+    const hipStream_t s = (hipStream_t)stream;
+    int st;
+    if (!p->d_Vrow) {
+        if ((st = setup_diag(p)) || (st = compute_homogeneous(p))) return st;
+    }
+    StageRecorder rec(p, s);
+    if ((st = forward_xy(p, ubar, p->spec.re, p->spec.im, p->K, s))) return st;
+    rec.mark("fft_forward");
+    DiagArgs a = diag_args(p);
+    a.adjoint = 1;
+    a.VinvT = p->d_Vrow;
+    a.VT = p->d_Vinvrow;
+    a.gbar_re = p->bc.re;
+    a.gbar_im = p->bc.im;
+    if ((st = launch_diag_modes(a, s))) return st;
+    rec.mark("diag_adjoint");
+    if ((st = inverse_xy(p, p->spec.re, p->spec.im, fbar, p->K, s, glbar, grbar))) return st;
+    rec.mark("fft_inverse");
+    if (lambda_bar) {
+        if (!p->d_partial && (st = dmalloc(&p->d_partial, 1025))) return st;
+        if ((st = launch_dot(fbar, u, (long long)p->nx * p->ny * p->K, p->d_partial, p->d_partial + 1024, s)))
+            return st;
+        HZ_HIP(hipMemcpyAsync(lambda_bar, p->d_partial + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+        HZ_HIP(hipStreamSynchronize(s));
+        rec.mark("lambda_dot");
+    }
     return HZ_OK;
 }
 

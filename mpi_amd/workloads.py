@@ -41,6 +41,23 @@ class SplitMix64:
         return math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(2.0 * math.pi * u2)
 
 
+def uniform_stream(count, seed):
+    """The first `count` uniforms of SplitMix64(seed), vectorised: the i-th
+    state is seed + (i+1) * gamma (mod 2^64), then the same finaliser."""
+    i = np.arange(1, count + 1, dtype=np.uint64)
+    z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def uniform_field(shape, lo, hi, seed):
+    """Seeded field with entries uniform in [lo, hi)."""
+    n = int(np.prod(shape))
+    return (lo + (hi - lo) * uniform_stream(n, seed)).reshape(shape)
+
+
 def legendre_table(zeta, lmax):
     """P_l(zeta) for l = 0..lmax, shape [lmax+1, len(zeta)] (three-term recurrence)."""
     P = np.zeros((lmax + 1, len(zeta)))

@@ -1,0 +1,188 @@
+"""Adjoint of the solve map (hz_adjoint) and its uses.
+
+Pins, from strongest to weakest:
+  * the oracle's solve map assembled column by column on a tiny grid: the
+    GPU adjoint must equal S^T ubar (S from the oracle's unit responses),
+  * the dot-product identity <S x, ubar> = <x, S^T ubar> at the benchmark
+    size, for random x and ubar,
+  * central finite differences in lambda, through torch.autograd.gradcheck,
+    and through 64 checkpointed Allen-Cahn steps (d/du0 and d/deps).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+ROBIN = (0.7, 1.3, 0.4, 2.0)
+DIRICHLET = (0.0, 1.0, 0.0, 1.0)
+NEUMANN = (1.0, 0.0, 1.0, 0.0)
+
+
+def make_plan(nx, ny, N, bc, lam, form, yr=3.0):
+    from mpi_amd.solver import HelmholtzPlan
+    al, bl, ar, br = bc
+    return HelmholtzPlan(nx, ny, N, 1, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2, lam=lam,
+                         alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form=form)
+
+
+def oracle_matrix(nx, ny, N, bc, lam, yr=3.0):
+    """Dense solve map of the oracle: columns = responses to unit inputs
+    (f nodes, then g_l, then g_r)."""
+    al, bl, ar, br = bc
+    desc = po.make_desc(nx, ny, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2,
+                        alphal=al, betal=bl, alphar=ar, betar=br, lam=lam)
+    K = po.total_z_points(desc)
+    nf, ng = nx * ny * K, nx * ny
+    S = np.zeros((nf, nf + 2 * ng))
+    zf, zg = np.zeros(nf), np.zeros(ng)
+    for c in range(nf + 2 * ng):
+        f, gl, gr = zf.copy(), zg.copy(), zg.copy()
+        if c < nf:
+            f[c] = 1.0
+        elif c < nf + ng:
+            gl[c - nf] = 1.0
+        else:
+            gr[c - nf - ng] = 1.0
+        S[:, c] = po.solve(desc, f, gl, gr).ravel()
+    return S, K
+
+
+@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("bc", [DIRICHLET, ROBIN, NEUMANN], ids=["dirichlet", "robin", "neumann"])
+def test_adjoint_equals_oracle_transpose(cuda, form, bc):
+    nx, ny, N, lam = 4, 8, 9, 3.0
+    S, K = oracle_matrix(nx, ny, N, bc, lam)
+    rng = np.random.default_rng(11)
+    ubar = rng.standard_normal((nx, ny, K))
+    want = S.T @ ubar.ravel()
+    plan = make_plan(nx, ny, N, bc, lam, form)
+    fbar, glbar, grbar, _ = plan.adjoint(dev(ubar))
+    got = np.concatenate([fbar.cpu().numpy().ravel(), glbar.cpu().numpy().ravel(), grbar.cpu().numpy().ravel()])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    # f at the element end nodes does not enter the solve: exact zeros
+    fb = fbar.cpu().numpy()
+    assert np.all(fb[:, :, 0] == 0.0) and np.all(fb[:, :, -1] == 0.0)
+    plan.close()
+
+
+@pytest.mark.parametrize("form,nx,N", [("diag", 256, 129), ("plain", 32, 33), ("diag", 64, 258)])
+def test_dot_product_identity(cuda, form, nx, N):
+    import torch
+    plan = make_plan(nx, nx, N, ROBIN, 10.0, form, yr=2 * math.pi)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    K = plan.nz
+    f = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    gl = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    gr = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    ubar = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    f[:, :, 0] = 0.0  # the solve ignores f on the end nodes
+    f[:, :, -1] = 0.0
+    u = plan.solve(f, gl, gr)
+    fbar, glbar, grbar, _ = plan.adjoint(ubar)
+    lhs = float(torch.sum(u * ubar))
+    rhs = float(torch.sum(f * fbar) + torch.sum(gl * glbar) + torch.sum(gr * grbar))
+    scale = float(torch.sum(torch.abs(u * ubar)))
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    plan.close()
+
+
+def test_lambda_gradient_matches_finite_differences(cuda):
+    import torch
+    nx, N, lam = 64, 65, 7.0
+    plan = make_plan(nx, nx, N, ROBIN, lam, "diag", yr=2 * math.pi)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    K = plan.nz
+    f = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    gl = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    gr = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    ubar = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+
+    def L(lv):
+        plan.set_lambda(lv)
+        return float(torch.sum(ubar * plan.solve(f, gl, gr)))
+
+    h = 1e-4 * lam
+    fd = (L(lam + h) - L(lam - h)) / (2 * h)
+    plan.set_lambda(lam)
+    u = plan.solve(f, gl, gr)
+    _, _, _, lam_bar = plan.adjoint(ubar, u)
+    assert abs(lam_bar - fd) <= 1e-6 * abs(fd)
+    plan.close()
+
+
+def test_autograd_gradcheck(cuda):
+    import torch
+    from mpi_amd.autograd import helmholtz_solve
+    plan = make_plan(4, 4, 7, ROBIN, 2.0, "diag")
+    K = plan.nz
+    g = torch.Generator(device="cuda").manual_seed(3)
+    f = torch.randn((4, 4, K), dtype=torch.float64, device="cuda", generator=g, requires_grad=True)
+    gl = torch.randn((4, 4), dtype=torch.float64, device="cuda", generator=g, requires_grad=True)
+    gr = torch.randn((4, 4), dtype=torch.float64, device="cuda", generator=g, requires_grad=True)
+    lam = torch.tensor(2.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda a, b, c, d: helmholtz_solve(a, b, c, d, plan), (f, gl, gr, lam),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6)
+    plan.close()
+
+
+def test_adjoint_rejects_unsupported(cuda):
+    import torch
+    from mpi_amd._lib import HzError
+    from mpi_amd.solver import HelmholtzPlan
+    plan = HelmholtzPlan(8, 8, 9, 2, form="diag")
+    ubar = torch.zeros((8, 8, plan.nz), dtype=torch.float64, device="cuda")
+    with pytest.raises(HzError):
+        plan.adjoint(ubar)
+    plan.close()
+
+
+def _allen_cahn_case(nx, N, steps, every):
+    import torch
+    from mpi_amd import workloads as wl
+    from mpi_amd.allen_cahn import AllenCahn
+    ac = AllenCahn(nx=nx, ny=nx, n=N, eps=0.02, dt=1e-3, steps=steps, checkpoint_every=every)
+    u0 = dev(wl.uniform_field((nx, nx, ac.K), -0.05, 0.05, seed=4))
+    target = dev(wl.uniform_field((nx, nx, ac.K), -1.0, 1.0, seed=5))
+    return ac, u0, target
+
+
+@pytest.mark.parametrize("nx,N,steps", [(32, 17, 16), (128, 65, 64)])
+def test_allen_cahn_gradients_match_finite_differences(cuda, nx, N, steps):
+    import torch
+    ac, u0, target = _allen_cahn_case(nx, N, steps, 8)
+    eps = 0.02
+    L, du0, deps = ac.loss_and_grad(u0, eps, target)
+    assert abs(L - ac.loss(ac.run(u0, eps), target)) <= 1e-13 * abs(L)
+    # d/deps
+    h = 1e-6
+    fd_eps = (ac.loss(ac.run(u0, eps + h), target) - ac.loss(ac.run(u0, eps - h), target)) / (2 * h)
+    assert abs(deps - fd_eps) <= 1e-6 * abs(fd_eps)
+    # directional derivative in u0
+    g = torch.Generator(device="cuda").manual_seed(7)
+    d = torch.rand(u0.shape, dtype=torch.float64, device="cuda", generator=g) - 0.5
+    hu = 1e-4
+    fd_u = (ac.loss(ac.run(u0 + hu * d, eps), target) - ac.loss(ac.run(u0 - hu * d, eps), target)) / (2 * hu)
+    ad_u = float(torch.sum(du0 * d))
+    assert abs(ad_u - fd_u) <= 1e-6 * abs(fd_u)
+    ac.close()
+
+
+def test_allen_cahn_checkpointing_is_exact(cuda):
+    """Recomputing segments from checkpoints reproduces the stored-trajectory gradient bit for bit."""
+    import torch
+    ac, u0, target = _allen_cahn_case(32, 17, 16, 8)
+    L8, g8, e8 = ac.loss_and_grad(u0, 0.02, target)
+    ac.every = 1
+    L1, g1, e1 = ac.loss_and_grad(u0, 0.02, target)
+    assert L8 == L1 and e8 == e1 and torch.equal(g8, g1)
+    ac.close()

@@ -1,0 +1,45 @@
+"""The drop-in C++ interface (include/hz_compat) driven by a caller written
+like the reference's own test program, TestNonPeriodic3D_MPI.cpp.
+
+The reference prints Error = 1.0891288e-13 for this problem at np = 1
+(same discretisation: 25 elements of N = 10)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(__file__), "..", "mpi_amd", "compat", "TestNonPeriodic3D")
+
+
+def run(*args):
+    return subprocess.run([EXE, *args], capture_output=True, text=True, timeout=120)
+
+
+def errors(out):
+    err = float(re.search(r"^Error = (\S+)", out, re.M).group(1))
+    derr = float(re.search(r"SecondDerivativeZ error = (\S+)", out).group(1))
+    return err, derr
+
+
+def test_shipped_problem_through_dropin(cuda):
+    r = run()
+    assert r.returncode == 0, r.stdout + r.stderr
+    err, derr = errors(r.stdout)
+    assert err <= 2.0e-13, err           # reference: 1.0891288e-13
+    # d2/dz2 of the computed solution vs -(4 pi)^2 u_exact (spectral accuracy, N = 10)
+    assert derr <= 1e-4 * 16 * 3.14159 ** 2, derr
+
+
+def test_callback_with_general_operator(cuda):
+    r = run("full")
+    assert r.returncode == 0, r.stdout + r.stderr
+    err, _ = errors(r.stdout)
+    assert err <= 2.0e-13
+
+
+def test_singular_operator_exits_like_reference(cuda):
+    r = run("singular")
+    assert r.returncode == 1
+    assert "ERROR: Matrix is singular!" in r.stdout
