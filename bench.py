@@ -64,6 +64,25 @@ def fft_bytes(nx, ny, K):
     return 2 * (real + half) + 2 * (2 * half)
 
 
+STAGE_KERNELS = {"diag_modes": "diag_modes_kernel", "lu_modes": "lu_modes_kernel",
+                 "fft_forward": "fft_lines", "fft_inverse": "fft_lines"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_256x256x129.json")
+
+
+def measured_traffic(stage):
+    """HBM bytes per dispatch of the stage's kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (None if absent)."""
+    try:
+        with open(TRAFFIC_FILE) as fp:
+            kernels = json.load(fp)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    prefix = STAGE_KERNELS.get(stage, stage)
+    cands = [(v["dispatches"], v["hbm_bytes_per_dispatch"]) for k, v in kernels.items()
+             if k.startswith(prefix) and "true>" not in k and v.get("hbm_bytes_per_dispatch")]
+    return max(cands)[1] if cands else None
+
+
 def make_inputs(plan):
     import numpy as np
     from mpi_amd import workloads as wl
@@ -109,6 +128,62 @@ def time_form(form, f, gl, gr, steps, warmup):
            "stage_ms": stages, "rel_err_vs_reference": err, "profiled_solves": nrec}
     plan.close()
     return res
+
+
+def time_adjoint(f, gl, gr, steps, warmup, fwd_ms):
+    """Adjoint of the chosen (DIAG) solve at the same size: dL/df, dL/dg from
+    a seeded dL/du, timed like the forward; then once more with dL/dlambda
+    (its scalar comes back to the host, so that call synchronises)."""
+    import torch
+    from mpi_amd.solver import HelmholtzPlan
+    plan = HelmholtzPlan(NX, NY, N, 1, lam=LAM, form="diag")
+    u = plan.solve(torch.from_numpy(f).cuda(), torch.from_numpy(gl).cuda(), torch.from_numpy(gr).cuda())
+    g = torch.Generator(device="cuda").manual_seed(3)
+    ubar = torch.randn(u.shape, dtype=torch.float64, device="cuda", generator=g)
+    for _ in range(warmup):
+        plan.adjoint(ubar)
+    torch.cuda.synchronize()
+    plan.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.adjoint(ubar)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages, _ = plan.profile_read()
+    plan.profile(False)
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        plan.adjoint(ubar, u)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    plan.close()
+    ms = 1e3 * (t1 - t0) / steps
+    return {"ms": ms, "ms_with_lambda_bar": 1e3 * (t3 - t2) / steps, "vs_forward": ms / fwd_ms,
+            "stage_ms": stages}
+
+
+def time_allen_cahn():
+    """Differentiation workload: 64 semi-implicit Allen-Cahn steps on
+    128 x 128 x 65 (eps = 0.02, dt = 1e-3), loss gradient w.r.t. u0 and eps
+    with a checkpoint every 8 steps."""
+    import torch
+    from mpi_amd import workloads as wl
+    from mpi_amd.allen_cahn import AllenCahn
+    ac = AllenCahn(nx=128, ny=128, n=65, eps=0.02, dt=1e-3, steps=64, checkpoint_every=8)
+    u0 = torch.from_numpy(wl.uniform_field((128, 128, ac.K), -0.05, 0.05, seed=4)).cuda()
+    target = torch.from_numpy(wl.uniform_field((128, 128, ac.K), -1.0, 1.0, seed=5)).cuda()
+    ac.loss_and_grad(u0, 0.02, target)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uT = ac.run(u0, 0.02)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L, du0, deps = ac.loss_and_grad(u0, 0.02, target)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ac.close()
+    return {"workload": "allen_cahn 128x128x65, 64 steps, checkpoint every 8", "forward_ms": 1e3 * (t1 - t0),
+            "loss_and_grad_ms": 1e3 * (t2 - t1), "loss": L, "dL_deps": deps}
 
 
 def cpu_baseline(f, gl, gr):
@@ -171,6 +246,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--forms", default="plain,diag")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and Allen-Cahn lines")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (read + write), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -215,7 +291,7 @@ def main():
         roof = {"kernel": stage, "bound": "hbm", "achieved": work / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "work_per_launch": work, "work_unit": "bytes"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(stage)
     roof["kernel_ms"] = ms
 
     line = {
@@ -232,6 +308,9 @@ def main():
         "chosen_vs_plain": (chosen["solves_per_s"] / results["plain"]["solves_per_s"]) if "plain" in results else None,
         "roofline": roof,
     }
+    if not args.no_extras and "diag" in results:
+        line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
+        line["allen_cahn"] = time_allen_cahn()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(f, gl, gr)
     print(json.dumps(line))

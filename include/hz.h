@@ -59,8 +59,10 @@ enum {
  *  DIAG      : z-diagonalised form -- Dzz_int = V diag(mu) V^-1 is computed
  *              once per plan; a solve is two FP64 MFMA contractions per
  *              column batch plus an O(n) shift per column.  No factorisation.
- *  MIXED     : FP32 LU per operator (FP64 LU for ill-conditioned low-|k|
- *              operators) with FP64 residuals and FP64 iterative refinement.
+ *  MIXED     : FP32 solution operator per (|kx|,|ky|) mode (FP64 for the
+ *              ill-conditioned low-|k| modes), formed once per lambda; every
+ *              solve applies it in FP32 and refines with FP64 residuals of the
+ *              GLL operator (refine_sweeps sweeps).  Transforms stay FP64.
  */
 enum { HZ_FORM_PLAIN_LU = 0, HZ_FORM_DIAG = 1, HZ_FORM_MIXED = 2 };
 
@@ -73,9 +75,11 @@ typedef struct {
     double lambda;           /* Helmholtz shift (reference Lambda)          */
     double alpha_l, beta_l, alpha_r, beta_r; /* Robin coefficients          */
     int form;                /* HZ_FORM_*                                   */
-    int refine_sweeps;       /* MIXED only: refinement sweeps (0 = default 2) */
-    double mixed_fp64_cond;  /* MIXED only: sigma below which an operator keeps
-                                an FP64 factorisation (0 = built-in rule)   */
+    int refine_sweeps;       /* MIXED only: refinement sweeps (0 = default 3,
+                                < 0 = none, <= 16)                          */
+    double mixed_fp64_cond;  /* MIXED only: shift sigma below which a mode keeps
+                                an FP64 operator (0 = built-in rule:
+                                cond * 2^-24 > 5e-2)                        */
 } hz_plan_desc;
 
 /* Replaces CreatePDEPlan (NonPeriodic3D_MPI.cpp:40-134): GLL nodes, scaled
@@ -167,9 +171,14 @@ int hz_batch_solve(hz_batch *batch, const double *const *f, const double *const 
                    const double *const *gr, double *const *u, hz_stream_t stream);
 int hz_batch_destroy(hz_batch *batch);
 
-/* Per-sweep refinement history of the last MIXED solve: max-norm of the
- * z-residual relative to the right-hand side after each sweep (host). */
+/* Refinement history of the last MIXED solve (synchronises the device):
+ * rel_residual[s] = max over modes and columns of |b - A x|_max / |b|_max
+ * after the FP32 solve (s = 0) and after each refinement sweep (s = 1..);
+ * *sweeps = number of values written (refine_sweeps + 1, at most max_sweeps).
+ * No counterpart in the reference (its LU is FP64 only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
+/* Number of modes of a MIXED plan that keep an FP64 operator. */
+int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 
 /* Host-only utilities (no device needed): the plan-time setup that
  * hz_plan_create performs, exposed so it can be checked on any machine.

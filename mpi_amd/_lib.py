@@ -60,6 +60,7 @@ SIGNATURES = {
                                       ctypes.POINTER(_VP), _VP]),
     "hz_batch_destroy": (ctypes.c_int, [_VP]),
     "hz_mixed_history": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI]),
+    "hz_mixed_fp64_modes": (ctypes.c_int, [_VP, _PI]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),
     "hz_host_eigen": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD]),
     "hz_plan_profile": (ctypes.c_int, [_VP, ctypes.c_int]),

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
     __shared__ double row_sig[RT], row_zl[RT], row_zr[RT], row_d0[RT], row_dN[RT];
     __shared__ long long row_col[RT];  // global offset of the row's column start (element start)
     __shared__ double row_u0[ADJ ? RT : 1], row_uN[ADJ ? RT : 1];
+    __shared__ double part0[4][(GENERAL || ADJ) ? RT : 1], partN[4][(GENERAL || ADJ) ? RT : 1];
 
     const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
@@ -115,33 +116,67 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
     __syncthreads();  // every wave has read the F tile
 
     // ---- shift, interface sweep, assemble Z^T in the tile ----
+    // q = y / (mu - sigma) stays in registers; GENERAL needs r0|rN . q per
+    // row (end-point derivatives of the particular solution), ADJ needs
+    // w0|wN . q (the adjoint load on zl, zr).  Each wave reduces its columns
+    // over the 16 lanes of a row, then the four waves' partials are summed.
     double inv[RBK][JT][4];
+    double p0[RBK][4], pN[RBK][4];
+    const double *dv0 = ADJ ? a.w0 : a.r0, *dvN = ADJ ? a.wN : a.rN;
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) p0[rb][r] = pN[rb][r] = 0.0;
 #pragma unroll
     for (int rb = 0; rb < RBK; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (w * JT + jt) + li;
             const double mu = j < n ? a.mu[j] : 0.0;
+            const double v0 = (GENERAL || ADJ) && j < n ? dv0[j] : 0.0;
+            const double vN = (GENERAL || ADJ) && j < n ? dvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * rb + 4 * r + lk;
                 const double d = mu - row_sig[row];
                 const double iv = (j < n) ? 1.0 / d : 0.0;
                 inv[rb][jt][r] = iv;
-                if (GENERAL || ADJ) tile[row * LD + j] = acc[rb][jt][r] * iv;  // q
+                if (GENERAL || ADJ) {
+                    const double q = acc[rb][jt][r] * iv;
+                    p0[rb][r] = fma(v0, q, p0[rb][r]);
+                    pN[rb][r] = fma(vN, q, pN[rb][r]);
+                }
             }
         }
-    if constexpr (ADJ) {
+    if constexpr (GENERAL || ADJ) {
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) {
+                    p0[rb][r] += __shfl_xor(p0[rb][r], o);
+                    pN[rb][r] += __shfl_xor(pN[rb][r], o);
+                }
+                if (li == 0) {
+                    const int row = 16 * rb + 4 * r + lk;
+                    part0[w][row] = p0[rb][r];
+                    partN[w][row] = pN[rb][r];
+                }
+            }
         __syncthreads();
-        // (w0|wN * inv) . y: the adjoint load on zl, zr from the interior
-        for (int item = tid; item < 2 * RT; item += 256) {
-            const int row = item >> 1, which = item & 1;
-            const double *wv = which ? a.wN : a.w0;
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s = fma(wv[j], tile[row * LD + j], s);
-            if (which) row_dN[row] = s;
-            else row_d0[row] = s;
+        for (int row = tid; row < RT; row += 256) {
+            double s0 = 0.0, sN = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) {
+                s0 += part0[ww][row];
+                sN += partN[ww][row];
+            }
+            row_d0[row] = s0;
+            row_dN[row] = sN;
         }
+    }
+    if constexpr (ADJ) {
         __syncthreads();
         // transposed sweep (one element): zbar -> gbar, P0bar, PNbar
         for (int row = tid; row < RT; row += 256) {
@@ -171,16 +206,6 @@ __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
             row_zr[row] = PNb;
         }
     } else if (GENERAL) {
-        __syncthreads();
-        // end-point derivatives of the particular solutions: r0|rN . q
-        for (int item = tid; item < 2 * RT; item += 256) {
-            const int row = item >> 1, which = item & 1;
-            const double *rv = which ? a.rN : a.r0;
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s = fma(rv[j], tile[row * LD + j], s);
-            if (which) row_dN[row] = s;
-            else row_d0[row] = s;
-        }
         __syncthreads();
         // sweep over the elements of each real spectral column
         for (int sc = tid; sc < 2 * fc_per_tile; sc += 256) {

@@ -19,6 +19,8 @@
 namespace hz {
 namespace {
 
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
+
 constexpr int bitrev(int i, int bits)
 {
     int r = 0;
@@ -74,8 +76,17 @@ __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArg
     __shared__ double tile[L * W];
 
     const int w = threadIdx.x % W, q = threadIdx.x / W;
-    const int lane = blockIdx.x * W + w;
-    const int batch = blockIdx.y;
+    // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
+    // so XCD x is given the contiguous logical range [x*T/8, (x+1)*T/8) of
+    // tiles (lane block fastest).  Neighbouring lane blocks of one line batch
+    // share the cache lines their 8*W-byte row segments straddle (K is odd),
+    // and they now share one L2.
+    const int nbx = gridDim.x, total = nbx * gridDim.y;
+    const int id = blockIdx.x + nbx * blockIdx.y;
+    const int xcd = id % kXcds, full = total / kXcds, rem = total % kXcds;
+    const int logical = xcd * full + min(xcd, rem) + id / kXcds;
+    const int lane = (logical % nbx) * W + w;
+    const int batch = logical / nbx;
     const double2 *__restrict__ tw = a.tw;
     // lane source: the main field, or one of the extra single-lane fields
     const double *in_re = a.in_re, *in_im = a.in_im;

@@ -95,6 +95,33 @@ struct DiagArgs {
 int diag_padded(int n);
 int launch_diag_modes(const DiagArgs &a, hipStream_t s);
 
+// MIXED form (mixed.hip).
+struct MixedSetupArgs {
+    int n, NP, ldx, modes;
+    const double *shift, *mu, *w0, *wN;
+    const double *VT, *Vinv;    // [NP][NP]: VT[k][i] = V[i][k], Vinv row-major
+    const int *slot;            // [modes]: >= 0 FP32 slot, < 0 FP64 slot -(s+1)
+    float *X32;                 // [n32][n][ldx]  X^T, i fastest
+    double *X64;                // [n64][n][ldx]
+    double *UL, *UR;            // [modes][n]
+};
+struct MixedArgs {
+    int nx, M, K, N, n, ne, ldx, sweeps, modes;
+    double *spec;               // real plane; imaginary plane at spec + plane_off
+    long long plane_off;
+    const double *bc_re, *bc_im, *shift;
+    const double *DzzT;         // [n][n], DzzT[j*n + i] = Dzz[i+1][j+1]
+    const double *Dz;           // [N][N]
+    const float *X32;
+    const double *X64;
+    const int *slot;
+    const double *UL, *UR, *hom;
+    double alpha_l, beta_l, alpha_r, beta_r;
+    double *hist;               // [(sweeps+1)][modes]
+};
+int launch_mixed_setup(const MixedSetupArgs &a, hipStream_t s);
+int launch_mixed_modes(const MixedArgs &a, hipStream_t s);
+
 struct LuFactorArgs {
     double *A;
     int n, count;
@@ -129,6 +156,12 @@ struct hz_plan {
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
     double *d_partial = nullptr;                      // reduction scratch
+    // MIXED form
+    int ldx = 0, n32 = 0, n64 = 0, sweeps = 0;
+    double condV = 0.0;
+    float *d_X32 = nullptr;
+    double *d_X64 = nullptr, *d_UL = nullptr, *d_UR = nullptr, *d_hist = nullptr;
+    int *d_slot = nullptr;
     // stage profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;      // events in use, per solve: nst+1

@@ -15,8 +15,4 @@ int hz_batch_solve(hz_batch *, const double *const *, const double *const *, con
     return set_error(HZ_ERR_UNSUPPORTED, "hz_batch_solve: not built yet");
 }
 int hz_batch_destroy(hz_batch *) { return HZ_OK; }
-int hz_mixed_history(const hz_plan *, double *, int, int *)
-{
-    return set_error(HZ_ERR_UNSUPPORTED, "hz_mixed_history: not built yet");
-}
 }

@@ -75,8 +75,10 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
     for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
-                      p->d_Vinvrow, p->d_partial})
+                      p->d_Vinvrow, p->d_partial, p->d_X64, p->d_UL, p->d_UR, p->d_hist})
         (void)hipFree(q);
+    (void)hipFree(p->d_X32);
+    (void)hipFree(p->d_slot);
     (void)hipFree(p->d_status);
     delete p;
 }
@@ -108,6 +110,102 @@ int compute_homogeneous(hz_plan *p)
     return HZ_OK;
 }
 
+// MIXED form: which modes keep FP64 operators, and the operators themselves.
+// cond(Dzz_int - sigma I) <= cond(V) max|mu - sigma| / min|mu - sigma|; FP32
+// refinement contracts the error by at most about cond * 2^-24 per sweep
+// (the measured contraction is far smaller), so modes above kMixedRate keep
+// FP64 (or, if the caller gave one, modes whose shift is below
+// desc.mixed_fp64_cond).
+constexpr double kMixedRate = 5e-2;
+
+int mixed_operators(hz_plan *p)
+{
+    const int n = p->n;
+    std::vector<int> slot(p->modes);
+    int n32 = 0, n64 = 0;
+    for (int m = 0; m < p->modes; ++m) {
+        const double sig = p->shift[m];
+        bool fp64;
+        if (p->d.mixed_fp64_cond > 0.0) {
+            fp64 = sig < p->d.mixed_fp64_cond;
+        } else {
+            double lo = INFINITY, hi = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double d = std::fabs(p->mu[k] - sig);
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
+            }
+            fp64 = !(p->condV * hi / lo * 0x1p-24 <= kMixedRate);
+        }
+        slot[m] = fp64 ? -(++n64) : n32++;
+    }
+    const size_t per = (size_t)n * p->ldx;
+    if (n32 != p->n32) {
+        (void)hipFree(p->d_X32);
+        p->d_X32 = nullptr;
+        p->n32 = 0;
+        int st = dmalloc(&p->d_X32, per * n32);
+        if (st) return st;
+        p->n32 = n32;
+    }
+    if (n64 != p->n64) {
+        (void)hipFree(p->d_X64);
+        p->d_X64 = nullptr;
+        p->n64 = 0;
+        int st = dmalloc(&p->d_X64, per * n64);
+        if (st) return st;
+        p->n64 = n64;
+    }
+    HZ_HIP(hipMemcpy(p->d_slot, slot.data(), sizeof(int) * p->modes, hipMemcpyHostToDevice));
+    MixedSetupArgs a{};
+    a.n = n;
+    a.NP = p->NP;
+    a.ldx = p->ldx;
+    a.modes = p->modes;
+    a.shift = p->d_shift;
+    a.mu = p->d_mu;
+    a.w0 = p->d_w0;
+    a.wN = p->d_wN;
+    a.VT = p->d_VT;
+    a.Vinv = p->d_Vinvrow;
+    a.slot = p->d_slot;
+    a.X32 = p->d_X32;
+    a.X64 = p->d_X64;
+    a.UL = p->d_UL;
+    a.UR = p->d_UR;
+    int st = launch_mixed_setup(a, nullptr);
+    if (st) return st;
+    HZ_HIP(hipDeviceSynchronize());
+    return HZ_OK;
+}
+
+int setup_mixed(hz_plan *p)
+{
+    p->ldx = (p->n + 3) / 4 * 4;
+    p->sweeps = p->d.refine_sweeps > 0 ? p->d.refine_sweeps : (p->d.refine_sweeps < 0 ? 0 : 3);
+    if (p->sweeps > 16) return set_error(HZ_ERR_ARG, "refine_sweeps must be <= 16");
+    if (p->ne > 4) return set_error(HZ_ERR_UNSUPPORTED, "MIXED form: at most 4 elements");
+    // 1-norm condition number of V (plan time, host)
+    const int n = p->n;
+    double nv = 0.0, ni = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double cv = 0.0, ci = 0.0;
+        for (int i = 0; i < n; ++i) {
+            cv += std::fabs(p->V[(size_t)i * n + j]);
+            ci += std::fabs(p->Vinv[(size_t)i * n + j]);
+        }
+        nv = std::max(nv, cv);
+        ni = std::max(ni, ci);
+    }
+    p->condV = nv * ni;
+    int st;
+    if ((st = dmalloc(&p->d_slot, p->modes)) || (st = dmalloc(&p->d_UL, (size_t)p->modes * n)) ||
+        (st = dmalloc(&p->d_UR, (size_t)p->modes * n)) ||
+        (st = dmalloc(&p->d_hist, (size_t)(p->sweeps + 1) * p->modes)))
+        return st;
+    return HZ_OK;
+}
+
 int compute_shifts(hz_plan *p)
 {
     p->shift.resize(p->modes);
@@ -115,7 +213,10 @@ int compute_shifts(hz_plan *p)
         for (int jw = 0; jw < p->M; ++jw)
             p->shift[(size_t)iw * p->M + jw] = mode_shift(p->d.lambda, iw, jw, p->Lx, p->Ly);
     HZ_HIP(hipMemcpy(p->d_shift, p->shift.data(), sizeof(double) * p->modes, hipMemcpyHostToDevice));
-    return compute_homogeneous(p);
+    int st = compute_homogeneous(p);
+    if (st) return st;
+    if (p->d_slot) return mixed_operators(p);
+    return HZ_OK;
 }
 
 // Eigen-decomposition of the interior operator and the derived vectors of
@@ -390,7 +491,8 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
         (st = upload(&p->d_twx, twiddles(p->nx))) || (st = upload(&p->d_twy, twiddles(p->ny))) ||
         (st = dmalloc(&p->spec.re, 2 * specN)) || (st = dmalloc(&p->bc.re, 2 * bcN)) ||
         (st = dmalloc(&p->d_status, p->modes)) ||
-        (d.form == HZ_FORM_DIAG && (st = setup_diag(p))) || (st = compute_shifts(p))) {
+        ((d.form == HZ_FORM_DIAG || d.form == HZ_FORM_MIXED) && (st = setup_diag(p))) ||
+        (d.form == HZ_FORM_MIXED && (st = setup_mixed(p))) || (st = compute_shifts(p))) {
         free_plan(p);
         return st;
     }
@@ -491,6 +593,37 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         DiagArgs a = diag_args(p);
         st = launch_diag_modes(a, s);
         rec.mark("diag_modes");
+    } else if (p->d.form == HZ_FORM_MIXED) {
+        MixedArgs a{};
+        a.nx = p->nx;
+        a.M = p->M;
+        a.K = p->K;
+        a.N = p->N;
+        a.n = p->n;
+        a.ne = p->ne;
+        a.ldx = p->ldx;
+        a.sweeps = p->sweeps;
+        a.modes = p->modes;
+        a.spec = p->spec.re;
+        a.plane_off = p->spec.im - p->spec.re;
+        a.bc_re = p->bc.re;
+        a.bc_im = p->bc.im;
+        a.shift = p->d_shift;
+        a.DzzT = p->d_DzzT;
+        a.Dz = p->d_Dz;
+        a.X32 = p->d_X32;
+        a.X64 = p->d_X64;
+        a.slot = p->d_slot;
+        a.UL = p->d_UL;
+        a.UR = p->d_UR;
+        a.hom = p->d_hom;
+        a.alpha_l = p->d.alpha_l;
+        a.beta_l = p->d.beta_l;
+        a.alpha_r = p->d.alpha_r;
+        a.beta_r = p->d.beta_r;
+        a.hist = p->d_hist;
+        st = launch_mixed_modes(a, s);
+        rec.mark("mixed_modes");
     } else {
         st = set_error(HZ_ERR_UNSUPPORTED, "form not built yet");
     }
@@ -599,6 +732,30 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
         HZ_HIP(hipStreamSynchronize(s));
         rec.mark("lambda_dot");
     }
+    return HZ_OK;
+}
+
+int hz_mixed_history(const hz_plan *p, double *rel_residual, int max_sweeps, int *sweeps)
+{
+    if (!p || !rel_residual || !sweeps) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->d.form != HZ_FORM_MIXED) return set_error(HZ_ERR_UNSUPPORTED, "not a MIXED plan");
+    const int S = p->sweeps + 1;
+    std::vector<double> h((size_t)S * p->modes);
+    HZ_HIP(hipDeviceSynchronize());
+    HZ_HIP(hipMemcpy(h.data(), p->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < S && s < max_sweeps; ++s) {
+        double v = 0.0;
+        for (int m = 0; m < p->modes; ++m) v = std::max(v, h[(size_t)s * p->modes + m]);
+        rel_residual[s] = v;
+    }
+    *sweeps = std::min(S, max_sweeps);
+    return HZ_OK;
+}
+
+int hz_mixed_fp64_modes(const hz_plan *p, int *fp64_modes)
+{
+    if (!p || !fp64_modes) return set_error(HZ_ERR_ARG, "null argument");
+    *fp64_modes = p->n64;
     return HZ_OK;
 }
 

@@ -155,6 +155,11 @@ class HelmholtzPlan:
                                         max_sweeps, ctypes.byref(n)))
         return h[: n.value]
 
+    def mixed_fp64_modes(self):
+        k = ctypes.c_int(0)
+        check(self.lib.hz_mixed_fp64_modes(self.handle, ctypes.byref(k)))
+        return k.value
+
     def close(self):
         if self.handle:
             self.lib.hz_plan_destroy(self.handle)

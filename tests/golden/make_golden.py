@@ -8,7 +8,8 @@ SHA-256 of the generated input bytes) and the reference's outputs either in
 full (small cases, hex floats) or as a SHA-256 of the exact output bytes
 plus sampled values and norms (large cases).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # all but the largest case
+    python tests/golden/make_golden.py --config4     # 512 x 512 x 257 (minutes)
 """
 import hashlib
 import json
@@ -170,9 +171,23 @@ def config2_inputs(N, size, lam=10.0):
     return d, f, gl, gr
 
 
+def config4_inputs():
+    """512 x 512 x 257 (one element), lambda = 100, 256 random modes."""
+    d = po.make_desc(512, 512, 257, 257, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
+                     alphal=0, betal=1, alphar=0, betar=1, lam=100.0)
+    x, y, zeta = wl.grid(512, 512, po.z_grid(d))
+    f, gl, gr = wl.random_problem(x, y, zeta, seed=4, nmodes=256, kmax=32, lmax=64)
+    return d, f, gl, gr
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if "--config4" in sys.argv:  # minutes of CPU time: on request only
+        d, f, gl, gr = config4_inputs()
+        solve_case("config4_N257_np1", d, f, gl, gr,
+                   gen="random_problem(seed=4, nmodes=256, kmax=32, lmax=64) lam=100")
+        return
     with open(os.path.join(HERE, "gll.json"), "w") as fp:
         json.dump(gll_fixture(), fp, indent=0)
     with open(os.path.join(HERE, "lu.json"), "w") as fp:
