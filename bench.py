@@ -11,7 +11,7 @@ Forms timed (libhz.so through its C ABI):
   plain  -- the reference algorithm: every solve factors all 16,641 mode
             operators with the reference's pivoting (FP64).
   diag   -- z-diagonalised form (FP64 MFMA contractions, no factorisation).
-  mixed  -- FP32 LU + FP64 residual refinement.
+  mixed  -- FP32 solution operators + FP64 residual refinement (3 sweeps).
 The reported `value` is the chosen form's rate (the fastest form whose
 error against the oracle is <= 1e-10); every form's rate and error is listed.
 
@@ -186,6 +186,101 @@ def time_allen_cahn():
             "loss_and_grad_ms": 1e3 * (t2 - t1), "loss": L, "dL_deps": deps}
 
 
+def time_batch(steps=5):
+    """Many-problem workload: 512 independent problems, nx, ny uniform in
+    {32, 64, 128}, N odd uniform in [17, 129], lambda log-uniform in
+    [1e-2, 1e3], 16 random modes each; one hz_batch (autotuned, graph
+    replay) against the same problems solved one plan at a time."""
+    import torch
+    from mpi_amd import workloads as wl
+    from mpi_amd.solver import BatchSolver, HelmholtzPlan
+    specs = wl.batch_specs(512, seed=3)
+    plans, fs, gls, grs, us = [], [], [], [], []
+    for sp in specs:
+        p = HelmholtzPlan(sp["nx"], sp["ny"], sp["N"], 1, lam=sp["lam"], form="diag")
+        f, gl, gr = wl.batch_fields(sp, p.z_nodes() - 1.0)
+        plans.append(p)
+        fs.append(torch.from_numpy(f).cuda())
+        gls.append(torch.from_numpy(gl).cuda())
+        grs.append(torch.from_numpy(gr).cuda())
+        us.append(p.empty_field())
+    b = BatchSolver(plans)
+    b.solve(fs, gls, grs, us)  # autotune + graph capture
+    b.solve(fs, gls, grs, us)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.solve(fs, gls, grs, us)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    b.status()
+    for p, f, gl, gr, u in zip(plans, fs, gls, grs, us):
+        p.solve(f, gl, gr, out=u)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        for p, f, gl, gr, u in zip(plans, fs, gls, grs, us):
+            p.solve(f, gl, gr, out=u)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    batch_ms = 1e3 * (t1 - t0) / steps
+    seq_ms = 1e3 * (t3 - t2) / steps
+    out = {"workload": "512 problems, nx,ny in {32,64,128}, N odd in [17,129], lambda in [1e-2,1e3], DIAG",
+           "points": int(sum(p.nx * p.ny * p.nz for p in plans)), "batch_ms": batch_ms,
+           "problems_per_s": 512 / (batch_ms * 1e-3), "one_plan_at_a_time_ms": seq_ms,
+           "speedup": seq_ms / batch_ms, "tuning": b.tuning()}
+    b.close()
+    for p in plans:
+        p.close()
+    return out
+
+
+def time_precision(steps=5):
+    """Precision workload: 512 x 512 x 257 (N = 257), lambda = 100, 256 random
+    modes: MIXED (FP32 operators + FP64 refinement) against the FP64 DIAG form,
+    each against the reference's own output sample; MIXED error per sweep."""
+    import numpy as np
+    import torch
+    from mpi_amd import workloads as wl
+    from mpi_amd.solver import HelmholtzPlan
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_util import load, sample, sha
+    rec = load("solve_config4_N257_np1.json")
+    res = {"workload": "512x512x257 (N=257), lambda=100, 256 random modes"}
+    plan = HelmholtzPlan(512, 512, 257, 1, lam=100.0, form="diag")
+    x, y, zeta = plan.grid()
+    f, gl, gr = wl.random_problem(x, y, zeta, seed=4, nmodes=256, kmax=32, lmax=64)
+    if sha(f) != rec["sha_f"]:
+        return {"error": "input generator drifted from the reference sample"}
+    df, dgl, dgr = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (f, gl, gr))
+    del f, gl, gr
+
+    def run(p):
+        u = p.solve(df, dgl, dgr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            p.solve(df, dgl, dgr, out=u)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        got, want = sample(u.cpu().numpy(), rec)
+        return ms, float(np.abs(got - want).max() / rec["max_abs_u"])
+
+    ms, err = run(plan)
+    plan.close()
+    res["diag"] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms, "rel_err_vs_reference": err}
+    per_sweep = []
+    for sweeps in (-1, 1, 2, 3):
+        p = HelmholtzPlan(512, 512, 257, 1, lam=100.0, form="mixed", refine_sweeps=sweeps)
+        ms, err = run(p)
+        per_sweep.append({"sweeps": max(sweeps, 0), "ms_per_solve": ms, "rel_err_vs_reference": err,
+                          "rel_residual_history": p.mixed_history().tolist(),
+                          "fp64_modes": p.mixed_fp64_modes(), "modes": p.num_modes})
+        p.close()
+    res["mixed"] = per_sweep
+    return res
+
+
 def cpu_baseline(f, gl, gr):
     """The reference solver (built from its sources into oracle/_ref) on this
     host, one solve of the same workload at np = 1 (same discretisation as the
@@ -244,7 +339,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="plain,diag")
+    ap.add_argument("--forms", default="plain,diag,mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and Allen-Cahn lines")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -311,6 +406,8 @@ def main():
     if not args.no_extras and "diag" in results:
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
         line["allen_cahn"] = time_allen_cahn()
+        line["batch"] = time_batch()
+        line["precision"] = time_precision()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(f, gl, gr)
     print(json.dumps(line))

@@ -161,14 +161,32 @@ int hz_lu_factor_batched(double *A, int n, int count, int *piv, double *det,
 int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int count,
                         hz_stream_t stream);
 
-/* Many independent problems of uneven size in one launch sequence.
- * plans[p] are DIAG-form plans; f[p], gl[p], gr[p], u[p] device arrays in
- * each plan's layout.  The first call records a HIP graph for this exact
- * set of plans and buffers; later calls with the same arguments replay it. */
+/* Many independent problems of uneven size solved together (no
+ * counterpart in the reference, which solves one problem per process).
+ * plans[p] are DIAG-form plans (distinct; nx, ny in [16, 256]; N - 2 <= 128;
+ * ne <= 16); f[p], g_l[p], g_r[p], u[p] are device arrays in each plan's
+ * layout.  Each stage (transform passes, mode solve) is one launch per kind
+ * of task over all problems.  The first solve of a batch shape autotunes the
+ * variant (mode-solve tile height) and records a HIP graph of the winner;
+ * later solves replay it (re-reading the buffer table when the pointers
+ * change).  Variants 0-3 (one persistent fused launch over a
+ * dependency-ordered task queue) are built but switched off.  Enqueued on `stream`; a plan in a batch must not be
+ * used by another solve while the batch runs. */
 typedef struct hz_batch hz_batch;
 int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch);
 int hz_batch_solve(hz_batch *batch, const double *const *f, const double *const *gl,
                    const double *const *gr, double *const *u, hz_stream_t stream);
+/* Synchronises the batch and reports HZ_ERR_HIP if a dependency wait of the
+ * fused launch hit its spin limit (results then unreliable). */
+int hz_batch_status(hz_batch *batch);
+/* The autotuner's choice (variant index, fused or staged, tile rows,
+ * persistent work-groups per CU) and the measured ms of every variant. */
+int hz_batch_tuning(const hz_batch *batch, int *variant, int *fused, int *tile_rows,
+                    int *wgs_per_cu, double *variant_ms, int max_variants);
+/* Pins the variant (4 or 5; 0-3 are switched off and refused; -1 = autotune
+ * again on the next solve); drops the recorded graph.  For tests and tuning
+ * studies. */
+int hz_batch_force_variant(hz_batch *batch, int variant);
 int hz_batch_destroy(hz_batch *batch);
 
 /* Refinement history of the last MIXED solve (synchronises the device):

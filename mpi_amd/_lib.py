@@ -59,6 +59,9 @@ SIGNATURES = {
     "hz_batch_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                                       ctypes.POINTER(_VP), _VP]),
     "hz_batch_destroy": (ctypes.c_int, [_VP]),
+    "hz_batch_status": (ctypes.c_int, [_VP]),
+    "hz_batch_tuning": (ctypes.c_int, [_VP, _PI, _PI, _PI, _PI, _PD, ctypes.c_int]),
+    "hz_batch_force_variant": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_mixed_history": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI]),
     "hz_mixed_fp64_modes": (ctypes.c_int, [_VP, _PI]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),

@@ -1,0 +1,617 @@
+// batch.hip -- many independent DIAG-form problems of uneven size (nx, ny,
+// N, ne, lambda, faces) solved together.
+//
+// Every problem is the five-stage pipeline of a solve:
+//   0 R2C along y (+ faces)   1 C2C along x (+ faces)   2 diagonalised mode solve
+//   3 C2C^-1 along x          4 C2R along y
+// split into tasks: one line batch of a transform pass (all its z-lane
+// blocks, so no two work-groups share the cache lines of one row), or one
+// tile of spectral columns of the mode solve.
+//
+// Fused variant (built, switched off -- see kVariants): ONE persistent
+// launch.  Tasks are numbered stage-major
+// (every problem's stage 0, then every problem's stage 1, ...) and handed out
+// by an atomic counter.  A task of stage s waits until its problem's stage
+// s-1 has completed all its tasks (a per-problem counter, released with an
+// agent-scope fence after the task's stores, acquired before the task's
+// loads, so the cross-XCD L2s are written back / invalidated).  Everything a
+// task waits for was handed out earlier to a running work-group, so the wait
+// always ends; a spin limit still bounds it and raises an error flag.
+// Staged variant: one launch per stage and task kind (the launch boundary
+// is the dependency), the same task bodies.
+//
+// The autotuner times the enabled variants (mode-solve tile height 32 or 64
+// rows) on the first solve of a batch shape, keeps the fastest, and captures
+// it in a HIP graph that later solves replay.
+#include "diag_body.h"
+#include "fft_body.h"
+
+#include <array>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace hz {
+
+constexpr int kStages = 5;
+
+struct BatchProblem {
+    FftArgs f[4];    // passes 0..3 of xy_pass_args
+    DiagArgs d;
+    int ly, lx;      // transform lengths along y, x
+    int ntask[kStages];
+    int dkind;       // 2 * (NJ == 8) + general
+};
+
+namespace {
+
+// ---- task bodies -------------------------------------------------------------
+template <int KIND>
+__device__ __noinline__ void fft_rows(const FftArgs &a, int L, int batch, double *smem)
+{
+    const int lanes = a.lanes + a.xlanes;
+    switch (L) {
+    case 16:
+        for (int lb = 0; lb * 64 < lanes; ++lb) fft_tile<16, 4, 4, 64, KIND>(a, lb, batch, smem);
+        break;
+    case 32:
+        for (int lb = 0; lb * 32 < lanes; ++lb) fft_tile<32, 8, 4, 32, KIND>(a, lb, batch, smem);
+        break;
+    case 64:
+        for (int lb = 0; lb * 32 < lanes; ++lb) fft_tile<64, 8, 8, 32, KIND>(a, lb, batch, smem);
+        break;
+    case 128:
+        for (int lb = 0; lb * 16 < lanes; ++lb) fft_tile<128, 16, 8, 16, KIND>(a, lb, batch, smem);
+        break;
+    case 256:
+        for (int lb = 0; lb * 16 < lanes; ++lb) fft_tile<256, 16, 16, 16, KIND>(a, lb, batch, smem);
+        break;
+    default:
+        break;
+    }
+}
+
+template <int RT>
+__device__ __noinline__ void diag_run(const DiagArgs &a, int kind, int tile, double *smem)
+{
+    switch (kind) {
+    case 0: diag_tile<4, RT, false, false>(a, tile, *reinterpret_cast<DiagShared<4, RT, false, false> *>(smem)); break;
+    case 1: diag_tile<4, RT, true, false>(a, tile, *reinterpret_cast<DiagShared<4, RT, true, false> *>(smem)); break;
+    case 2: diag_tile<8, RT, false, false>(a, tile, *reinterpret_cast<DiagShared<8, RT, false, false> *>(smem)); break;
+    case 3: diag_tile<8, RT, true, false>(a, tile, *reinterpret_cast<DiagShared<8, RT, true, false> *>(smem)); break;
+    default: break;
+    }
+}
+
+template <int RT>
+__device__ void run_task(const BatchProblem &pr, int s, int local, double *smem)
+{
+    switch (s) {
+    case 0: fft_rows<FFT_R2C>(pr.f[0], pr.ly, local, smem); break;
+    case 1: fft_rows<FFT_C2C_FWD>(pr.f[1], pr.lx, local, smem); break;
+    case 2: diag_run<RT>(pr.d, pr.dkind, local, smem); break;
+    case 3: fft_rows<FFT_C2C_INV>(pr.f[2], pr.lx, local, smem); break;
+    case 4: fft_rows<FFT_C2R>(pr.f[3], pr.ly, local, smem); break;
+    default: break;
+    }
+}
+
+// ctl layout: [0] task counter, [1] error flag, [2..3] unused,
+// [4 + p*kStages + s] completed tasks of stage s of problem p.
+__global__ void batch_reset_kernel(int *ctl, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0;
+}
+
+// Problem of task `rel` of stage s: off[s*(nprob+1) + p] is the first task
+// of problem p within stage s.
+__device__ __forceinline__ int find_problem(const int *__restrict__ off, int nprob, int s, int rel, int *local)
+{
+    const int *o = off + s * (nprob + 1);
+    int lo = 0, hi = nprob;  // o[lo] <= rel < o[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (o[mid] <= rel) lo = mid;
+        else hi = mid;
+    }
+    *local = rel - o[lo];
+    return lo;
+}
+
+// Fused persistent launch.  base[s]: first task of stage s (base[kStages] =
+// total); tasks are taken from the counter ctl[0].
+template <int RT>
+__global__ __launch_bounds__(256, 2) void batch_fused_kernel(const BatchProblem *__restrict__ P, int nprob,
+                                                             const int *__restrict__ off,
+                                                             const int *__restrict__ base, int *ctl, int spin_limit)
+{
+    extern __shared__ double smem[];
+    __shared__ int s_stage, s_p, s_local;
+    int *done = ctl + 4;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int s = -1, p = 0, local = 0;
+            const int t = atomicAdd(&ctl[0], 1);
+            if (t < base[kStages]) {
+                s = 0;
+                while (t >= base[s + 1]) ++s;
+                p = find_problem(off, nprob, s, t - base[s], &local);
+                if (s > 0) {
+                    const int need = P[p].ntask[s - 1];
+                    int *cnt = &done[p * kStages + s - 1];
+                    int spins = 0;
+                    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(4);
+                        if (++spins > spin_limit) {
+                            atomicOr(&ctl[1], 1);
+                            break;
+                        }
+                    }
+                }
+            }
+            s_stage = s;
+            s_p = p;
+            s_local = local;
+        }
+        __syncthreads();
+        const int s = s_stage;
+        if (s < 0) break;
+        // every wave orders its loads after the wait (cross-XCD caches)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int p = s_p;
+        run_task<RT>(P[p], s, s_local, smem);
+        // every wave's stores are complete and written back before the signal
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(&done[p * kStages + s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Staged launches: one launch per stage and kind of task (transform length,
+// or mode-solve variant), each work-group one task from a (problem, task)
+// list; the launch boundaries order the stages, and every kernel inlines only
+// its own body, so it gets its own register budget.
+template <int L, int R1, int R2, int W, int KIND>
+__global__ __launch_bounds__(256) void batch_fft_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list,
+                                                      int pass)
+{
+    __shared__ double tile[L * W];
+    const int2 t = list[blockIdx.x];
+    const FftArgs &a = P[t.x].f[pass];
+    const int lanes = a.lanes + a.xlanes;
+    for (int lb = 0; lb * W < lanes; ++lb) fft_tile<L, R1, R2, W, KIND>(a, lb, t.y, tile);
+}
+
+template <int NJ, int RT, bool GENERAL>
+__global__ __launch_bounds__(256) void batch_diag_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list)
+{
+    __shared__ DiagShared<NJ, RT, GENERAL, false> sh;
+    const int2 t = list[blockIdx.x];
+    diag_tile<NJ, RT, GENERAL, false>(P[t.x].d, t.y, sh);
+}
+
+template <int RT>
+constexpr size_t batch_lds()
+{
+    constexpr size_t fft = sizeof(double) * 256 * 16;  // largest transform tile (L = 256, W = 16)
+    constexpr size_t dg = sizeof(DiagShared<8, RT, true, false>);
+    return fft > dg ? fft : dg;
+}
+
+}  // namespace
+}  // namespace hz
+
+using namespace hz;
+
+// ---- host side ---------------------------------------------------------------
+namespace {
+
+struct Variant {
+    int rt;       // diag tile rows
+    int fused;    // 1 persistent, 0 one launch per stage
+    int wgs;      // persistent work-groups per CU
+    int enabled;  // offered to the autotuner and to hz_batch_force_variant
+};
+
+// The fused persistent variants are switched off: their first run on the
+// GPU did not finish (its dependency waits ran to the spin limit, i.e. the
+// completion counters written by work-groups on other XCDs were not seen),
+// and the cause is not established.  Only the staged variants run.
+const Variant kVariants[] = {{32, 1, 2, 0}, {32, 1, 3, 0}, {32, 1, 4, 0}, {64, 1, 2, 0}, {32, 0, 0, 1}, {64, 0, 0, 1}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kSpinLimit = 1 << 16;  // ~0.1 s of 2-us probes, then the error flag
+
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tuned;  // batch shape -> variant
+
+}  // namespace
+
+struct hz_batch {
+    std::vector<hz_plan *> plans;
+    int n = 0, device = 0, cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    BatchProblem *d_probs = nullptr, *h_probs = nullptr;
+    int *d_tab[2] = {nullptr, nullptr};  // per tile height: off then base
+    std::vector<int> h_tab[2];
+    int *d_ctl = nullptr;
+    int nctl = 0;
+    // staged variant: per tile height, task lists grouped by (stage, kind)
+    int2 *d_list[2] = {nullptr, nullptr};
+    std::vector<std::array<int, 3>> groups[2];  // {stage, kind, first}, count in next first
+    std::vector<const void *> last;
+    std::string shape;
+    int variant = -1;
+    double variant_ms[kNumVariants] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int rt_index(int rt) { return rt == 64 ? 1 : 0; }
+
+// off/base tables of the batch for diag tiles of `rt` rows
+std::vector<int> task_table(const hz_batch *b, int rt, std::vector<std::array<int, kStages>> *counts)
+{
+    const int n = b->n;
+    std::vector<int> tab((size_t)kStages * (n + 1) + kStages + 1, 0);
+    counts->assign(n, {});
+    for (int p = 0; p < n; ++p) {
+        const hz_plan *pl = b->plans[p];
+        const int fc_per_tile = rt / (2 * pl->ne);
+        const int nfc = pl->nx * pl->M;
+        (*counts)[p] = {pl->nx, pl->M, (nfc + fc_per_tile - 1) / fc_per_tile, pl->M, pl->nx};
+    }
+    int total = 0;
+    int *base = tab.data() + (size_t)kStages * (n + 1);
+    for (int s = 0; s < kStages; ++s) {
+        base[s] = total;
+        int *o = tab.data() + (size_t)s * (n + 1);
+        int acc = 0;
+        for (int p = 0; p < n; ++p) {
+            o[p] = acc;
+            acc += (*counts)[p][s];
+        }
+        o[n] = acc;
+        total += acc;
+    }
+    base[kStages] = total;
+    return tab;
+}
+
+int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, const double *const *gr,
+                  double *const *u, int rt)
+{
+    std::vector<std::array<int, kStages>> counts;
+    (void)task_table(b, rt, &counts);
+    for (int p = 0; p < b->n; ++p) {
+        hz_plan *pl = b->plans[p];
+        BatchProblem &q = b->h_probs[p];
+        q = BatchProblem{};
+        q.f[0] = xy_pass_args(pl, 0, f[p], pl->spec.re, pl->spec.im, nullptr, pl->K, gl[p], gr[p], nullptr, nullptr);
+        q.f[1] = xy_pass_args(pl, 1, nullptr, pl->spec.re, pl->spec.im, nullptr, pl->K, gl[p], gr[p], nullptr,
+                              nullptr);
+        q.f[2] = xy_pass_args(pl, 2, nullptr, pl->spec.re, pl->spec.im, nullptr, pl->K, nullptr, nullptr, nullptr,
+                              nullptr);
+        q.f[3] = xy_pass_args(pl, 3, nullptr, pl->spec.re, pl->spec.im, u[p], pl->K, nullptr, nullptr, nullptr,
+                              nullptr);
+        q.d = diag_args(pl);
+        q.ly = pl->ny;
+        q.lx = pl->nx;
+        for (int s = 0; s < kStages; ++s) q.ntask[s] = counts[p][s];
+        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0;
+        q.dkind = 2 * (pl->n > 64) + (general ? 1 : 0);
+    }
+    return HZ_OK;
+}
+
+// Kind of a transform task: log2 of its length; of a mode-solve task: dkind.
+int task_kind(const BatchProblem &q, int s) { return s == 2 ? q.dkind : (s == 0 || s == 4 ? q.ly : q.lx); }
+
+template <int KIND>
+void launch_fft_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int L, int pass)
+{
+    switch (L) {
+    case 16: hipLaunchKernelGGL((batch_fft_list<16, 4, 4, 64, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
+    case 32: hipLaunchKernelGGL((batch_fft_list<32, 8, 4, 32, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
+    case 64: hipLaunchKernelGGL((batch_fft_list<64, 8, 8, 32, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
+    case 128: hipLaunchKernelGGL((batch_fft_list<128, 16, 8, 16, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
+    default: hipLaunchKernelGGL((batch_fft_list<256, 16, 16, 16, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
+    }
+}
+
+template <int RT>
+void launch_diag_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int kind)
+{
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((batch_diag_list<4, RT, false>), dim3(count), dim3(256), 0, s, P, list); break;
+    case 1: hipLaunchKernelGGL((batch_diag_list<4, RT, true>), dim3(count), dim3(256), 0, s, P, list); break;
+    case 2: hipLaunchKernelGGL((batch_diag_list<8, RT, false>), dim3(count), dim3(256), 0, s, P, list); break;
+    default: hipLaunchKernelGGL((batch_diag_list<8, RT, true>), dim3(count), dim3(256), 0, s, P, list); break;
+    }
+}
+
+template <int RT>
+int enqueue_t(hz_batch *b, const Variant &v, hipStream_t s)
+{
+    const int r = rt_index(RT);
+    const int *tab = b->d_tab[r];
+    const int *off = tab, *base = tab + (size_t)kStages * (b->n + 1);
+    constexpr size_t lds = batch_lds<RT>();
+    static bool attr_set = false;  // opt in to more than 64 KiB of dynamic LDS once
+    if (!attr_set) {
+        HZ_HIP(hipFuncSetAttribute((const void *)batch_fused_kernel<RT>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    if (v.fused) {
+        hipLaunchKernelGGL(batch_reset_kernel, dim3(8), dim3(256), 0, s, b->d_ctl, b->nctl);
+        hipLaunchKernelGGL((batch_fused_kernel<RT>), dim3(v.wgs * b->cus), dim3(256), lds, s, b->d_probs, b->n, off,
+                           base, b->d_ctl, kSpinLimit);
+    } else {
+        const auto &g = b->groups[r];
+        for (size_t i = 0; i + 1 < g.size(); ++i) {
+            const int st = g[i][0], kind = g[i][1], first = g[i][2], count = g[i + 1][2] - first;
+            const int2 *list = b->d_list[r] + first;
+            const int pass = st < 2 ? st : st - 1;
+            switch (st) {
+            case 0: launch_fft_list<FFT_R2C>(s, b->d_probs, list, count, kind, pass); break;
+            case 1: launch_fft_list<FFT_C2C_FWD>(s, b->d_probs, list, count, kind, pass); break;
+            case 2: launch_diag_list<RT>(s, b->d_probs, list, count, kind); break;
+            case 3: launch_fft_list<FFT_C2C_INV>(s, b->d_probs, list, count, kind, pass); break;
+            default: launch_fft_list<FFT_C2R>(s, b->d_probs, list, count, kind, pass); break;
+            }
+        }
+    }
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+// Task lists of the staged variant for tile height rt (problem kinds known
+// from the filled problem table).
+int build_lists(hz_batch *b, int rt)
+{
+    const int r = rt_index(rt);
+    std::vector<std::array<int, kStages>> counts;
+    (void)task_table(b, rt, &counts);
+    std::map<std::pair<int, int>, std::vector<int2>> by;
+    for (int p = 0; p < b->n; ++p)
+        for (int st = 0; st < kStages; ++st) {
+            auto &v = by[{st, task_kind(b->h_probs[p], st)}];
+            for (int t = 0; t < counts[p][st]; ++t) v.push_back(make_int2(p, t));
+        }
+    std::vector<int2> all;
+    b->groups[r].clear();
+    for (auto &kv : by) {  // ordered by stage, then kind
+        b->groups[r].push_back({kv.first.first, kv.first.second, (int)all.size()});
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+    }
+    b->groups[r].push_back({kStages, 0, (int)all.size()});
+    (void)hipFree(b->d_list[r]);
+    b->d_list[r] = nullptr;
+    if (hipMalloc((void **)&b->d_list[r], sizeof(int2) * all.size()) != hipSuccess)
+        return set_error(HZ_ERR_ALLOC, "batch allocation failed");
+    HZ_HIP(hipMemcpy(b->d_list[r], all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+    return HZ_OK;
+}
+
+int enqueue(hz_batch *b, int vi, hipStream_t s)
+{
+    const Variant &v = kVariants[vi];
+    return v.rt == 64 ? enqueue_t<64>(b, v, s) : enqueue_t<32>(b, v, s);
+}
+
+// Problem tables depend on the variant's tile height through the diag task
+// counts: refill and upload them for variant vi.
+int upload_problems(hz_batch *b, const double *const *f, const double *const *gl, const double *const *gr,
+                    double *const *u, int vi)
+{
+    HZ_HIP(hipStreamSynchronize(b->stream));  // the pinned table may still be in flight
+    fill_problems(b, f, gl, gr, u, kVariants[vi].rt);
+    HZ_HIP(hipMemcpyAsync(b->d_probs, b->h_probs, sizeof(BatchProblem) * b->n, hipMemcpyHostToDevice, b->stream));
+    return HZ_OK;
+}
+
+void free_batch(hz_batch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : {b->ev_in, b->ev_out, b->t0, b->t1})
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    (void)hipFree(b->d_probs);
+    (void)hipHostFree(b->h_probs);
+    for (int *t : b->d_tab) (void)hipFree(t);
+    for (int2 *l : b->d_list) (void)hipFree(l);
+    (void)hipFree(b->d_ctl);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
+{
+    if (!plans || !batch || count < 1) return set_error(HZ_ERR_ARG, "bad argument");
+    *batch = nullptr;
+    int dev = 0;
+    HZ_HIP(hipGetDevice(&dev));
+    std::string shape;
+    for (int p = 0; p < count; ++p) {
+        const hz_plan *pl = plans[p];
+        if (!pl) return set_error(HZ_ERR_ARG, "null plan");
+        for (int q = 0; q < p; ++q)
+            if (plans[q] == pl) return set_error(HZ_ERR_ARG, "a plan appears twice in the batch");
+        if (pl->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG-form plans only");
+        if (pl->device != dev) return set_error(HZ_ERR_ARG, "batch: plans on another device");
+        if (pl->n > 128) return set_error(HZ_ERR_UNSUPPORTED, "batch: N-2 must be <= 128");
+        if (pl->nx < 16 || pl->nx > 256 || pl->ny < 16 || pl->ny > 256)
+            return set_error(HZ_ERR_UNSUPPORTED, "batch: nx, ny must be in [16, 256]");
+        if (2 * pl->ne > 32) return set_error(HZ_ERR_UNSUPPORTED, "batch: at most 16 elements");
+        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0;
+        shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +
+                 std::to_string(pl->ne) + (general ? "g;" : ";");
+    }
+    hz_batch *b = new hz_batch();
+    b->plans.assign(plans, plans + count);
+    b->n = count;
+    b->device = dev;
+    b->shape = shape;
+    b->nctl = 4 + kStages * count;
+    int st = HZ_OK;
+    auto fail = [&](int code) {
+        free_batch(b);
+        return code;
+    };
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->cus < 1)
+        return fail(set_error(HZ_ERR_HIP, "cannot read the CU count"));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess)
+        return fail(set_error(HZ_ERR_HIP, "stream/event creation failed"));
+    if (hipMalloc((void **)&b->d_probs, sizeof(BatchProblem) * count) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_probs, sizeof(BatchProblem) * count) != hipSuccess ||
+        hipMalloc((void **)&b->d_ctl, sizeof(int) * b->nctl) != hipSuccess)
+        return fail(set_error(HZ_ERR_ALLOC, "batch allocation failed"));
+    for (int r = 0; r < 2; ++r) {
+        std::vector<std::array<int, kStages>> counts;
+        b->h_tab[r] = task_table(b, r ? 64 : 32, &counts);
+        if (hipMalloc((void **)&b->d_tab[r], sizeof(int) * b->h_tab[r].size()) != hipSuccess)
+            return fail(set_error(HZ_ERR_ALLOC, "batch allocation failed"));
+        if (hipMemcpy(b->d_tab[r], b->h_tab[r].data(), sizeof(int) * b->h_tab[r].size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return fail(set_error(HZ_ERR_HIP, "batch table upload failed"));
+    }
+    {
+        // kinds only depend on the plans: fill the table with null buffers
+        std::vector<const double *> nil(count, nullptr);
+        std::vector<double *> nilo(count, nullptr);
+        for (int r = 0; r < 2; ++r) {
+            fill_problems(b, nil.data(), nil.data(), nil.data(), nilo.data(), r ? 64 : 32);
+            if ((st = build_lists(b, r ? 64 : 32))) return fail(st);
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(shape);
+        if (it != g_tuned.end()) b->variant = it->second;
+    }
+    *batch = b;
+    return HZ_OK;
+}
+
+int hz_batch_solve(hz_batch *b, const double *const *f, const double *const *gl, const double *const *gr,
+                   double *const *u, hz_stream_t stream)
+{
+    if (!b || !f || !gl || !gr || !u) return set_error(HZ_ERR_ARG, "null argument");
+    for (int p = 0; p < b->n; ++p)
+        if (!f[p] || !gl[p] || !gr[p] || !u[p]) return set_error(HZ_ERR_ARG, "null buffer");
+    const hipStream_t cs = (hipStream_t)stream;
+    int st;
+    HZ_HIP(hipEventRecord(b->ev_in, cs));
+    HZ_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+    std::vector<const void *> ptrs;
+    for (int p = 0; p < b->n; ++p) {
+        ptrs.push_back(f[p]);
+        ptrs.push_back(gl[p]);
+        ptrs.push_back(gr[p]);
+        ptrs.push_back(u[p]);
+    }
+    if (b->variant < 0) {
+        // autotune: time every variant on these buffers (each run is a full solve)
+        double best = 1e300;
+        int bi = 0;
+        for (int vi = 0; vi < kNumVariants; ++vi) {
+            if (!kVariants[vi].enabled) continue;
+            if ((st = upload_problems(b, f, gl, gr, u, vi))) return st;
+            if ((st = enqueue(b, vi, b->stream))) return st;  // warm-up
+            HZ_HIP(hipEventRecord(b->t0, b->stream));
+            for (int r = 0; r < 3; ++r)
+                if ((st = enqueue(b, vi, b->stream))) return st;
+            HZ_HIP(hipEventRecord(b->t1, b->stream));
+            HZ_HIP(hipEventSynchronize(b->t1));
+            float ms = 0.0f;
+            HZ_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+            b->variant_ms[vi] = ms / 3.0;
+            if (ms < best) {
+                best = ms;
+                bi = vi;
+            }
+        }
+        b->variant = bi;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[b->shape] = bi;
+    }
+    if (ptrs != b->last) {
+        if ((st = upload_problems(b, f, gl, gr, u, b->variant))) return st;
+        b->last = ptrs;
+    }
+    if (!b->exec) {
+        HZ_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        st = enqueue(b, b->variant, b->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st) return st;
+        if (e != hipSuccess) return set_error(HZ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HZ_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HZ_HIP(hipGraphLaunch(b->exec, b->stream));
+    HZ_HIP(hipEventRecord(b->ev_out, b->stream));
+    HZ_HIP(hipStreamWaitEvent(cs, b->ev_out, 0));
+    return HZ_OK;
+}
+
+int hz_batch_status(hz_batch *b)
+{
+    if (!b) return set_error(HZ_ERR_ARG, "null argument");
+    HZ_HIP(hipStreamSynchronize(b->stream));
+    int flags[2] = {0, 0};
+    HZ_HIP(hipMemcpy(flags, b->d_ctl, sizeof(flags), hipMemcpyDeviceToHost));
+    if (flags[1]) return set_error(HZ_ERR_HIP, "batch: a dependency wait hit its spin limit");
+    return HZ_OK;
+}
+
+int hz_batch_tuning(const hz_batch *b, int *variant, int *fused, int *tile_rows, int *wgs_per_cu,
+                    double *variant_ms, int max_variants)
+{
+    if (!b) return set_error(HZ_ERR_ARG, "null argument");
+    const int vi = b->variant;
+    if (variant) *variant = vi;
+    if (vi >= 0) {
+        if (fused) *fused = kVariants[vi].fused;
+        if (tile_rows) *tile_rows = kVariants[vi].rt;
+        if (wgs_per_cu) *wgs_per_cu = kVariants[vi].wgs;
+    }
+    if (variant_ms)
+        for (int i = 0; i < kNumVariants && i < max_variants; ++i) variant_ms[i] = b->variant_ms[i];
+    return HZ_OK;
+}
+
+int hz_batch_force_variant(hz_batch *b, int variant)
+{
+    if (!b || variant < -1 || variant >= kNumVariants) return set_error(HZ_ERR_ARG, "bad variant");
+    if (variant >= 0 && !kVariants[variant].enabled)
+        return set_error(HZ_ERR_UNSUPPORTED, "batch: the fused persistent variants are switched off");
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->variant = variant;
+    b->last.clear();  // the problem table depends on the variant's tile height
+    return HZ_OK;
+}
+
+int hz_batch_destroy(hz_batch *b)
+{
+    free_batch(b);
+    return HZ_OK;
+}
+
+}  // extern "C"

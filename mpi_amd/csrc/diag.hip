@@ -1,323 +1,15 @@
-// diag.hip -- the z-diagonalised mode solve on FP64 matrix cores.
-//
-// The interior GLL operator of every element is the same matrix, so with
-// Dzz_int = V diag(mu) V^{-1} (computed once per plan, eig.cpp) the
-// reference's per-mode solve A(sigma)^{-1} = V (diag(mu) - sigma)^{-1} V^{-1}
-// needs no factorisation.  For each real spectral column (x index, y index,
-// re/im) and element e, with sigma = lambda + kx^2 + ky^2:
-//   y   = V^{-1} f_e                                  (MFMA contraction)
-//   inv = 1 / (mu - sigma),   q = y * inv
-//   p_e = V q  is the particular solution, ul = V (w0 * inv), ur = V (wN * inv)
-//   with w0 = V^{-1}(-Dzz[1..N-2][0]), wN = V^{-1}(-Dzz[1..N-2][N-1])
-//   (NonPeriodic3D_MPI.cpp:188-222); the end-point derivatives are
-//   Dz[0|N-1].p_e = r0|rN . q with r0|rN = Dz[0|N-1][1..N-2] V (:175-179);
-//   the element-interface sweep (:327-580) gives zl_e, zr_e and
-//   u_e = V ((y + zl_e w0 + zr_e wN) * inv)          (MFMA contraction)
-//   with u at the element ends equal to zl_e, zr_e (:582-587).
-//
-// Adjoint (ADJ = true, one element): the transposed column map.  With
-// ubar the column of dL/du, y = V^T ubar_int, inv as above,
-//   zbl = ubar_0 + (w0*inv).y,  zbr = ubar_{N-1} + (wN*inv).y
-// (the adjoints of zl, zr through u = p + zl ul + zr ur), the scalar sweep
-// run backwards gives dL/dg_l, dL/dg_r and the adjoints P0b, PNb of the
-// end-point derivatives, and
-//   fbar_int = V^{-T} ((y + P0b r0 + PNb rN) * inv),  fbar = 0 at the ends.
-// The kernel is the forward one with B1 = V, B2 = V^{-1}.
-//
-// A work-group owns RT tile rows (spectral column x element) and all NP
-// (padded n) eigen/grid indices.  Contractions are transposed so that MFMA
-// output lanes run along z, which is contiguous in memory:
-//   Y^T = F^T Vinv^T,   U^T = Z^T V^T
-// v_mfma_f64_16x16x4f64 operand layout (wave64, measured on gfx950):
-// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] (4 per lane) in
-// lane j + 16*(i%4), element i/4.  Each wave computes NJ/4 output column tiles for all RT rows,
-// reading A from the LDS tile and B (Vinv^T, V^T) from L2.
-#include "hz_internal.h"
+// diag.hip -- the z-diagonalised mode solve on FP64 matrix cores: launch
+// wrappers around the tile body in diag_body.h (scheme described there).
+#include "diag_body.h"
 
 namespace hz {
 namespace {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int WAVE = 64;
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
 template <int NJ, int RT, bool GENERAL, bool ADJ>
 __global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
 {
-    constexpr int NP = 16 * NJ;      // padded interior size
-    constexpr int LD = NP + 2;       // LDS row stride (doubles)
-    constexpr int JT = NJ / 4;       // output column tiles per wave
-    constexpr int RBK = RT / 16;     // row blocks
-    __shared__ double tile[RT * LD];
-    __shared__ double row_sig[RT], row_zl[RT], row_zr[RT], row_d0[RT], row_dN[RT];
-    __shared__ long long row_col[RT];  // global offset of the row's column start (element start)
-    __shared__ double row_u0[ADJ ? RT : 1], row_uN[ADJ ? RT : 1];
-    __shared__ double part0[4][(GENERAL || ADJ) ? RT : 1], partN[4][(GENERAL || ADJ) ? RT : 1];
-
-    const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
-    const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
-    const int rows_per_fc = 2 * E;                    // re/im x elements
-    const int fc_per_tile = RT / rows_per_fc;
-    const int fc0 = blockIdx.x * fc_per_tile;
-    const int nfc = a.nx * M;
-
-    // ---- row bookkeeping: row = (fc_local*2 + plane)*E + e ----
-    for (int row = tid; row < RT; row += 256) {
-        const int fl = row / rows_per_fc, rem = row % rows_per_fc;
-        const int plane = rem / E, e = rem % E;
-        const int fc = fc0 + fl;
-        double sig = 0.0;
-        long long col = -1;
-        if (fl < fc_per_tile && fc < nfc) {
-            const int i = fc / M, m = fc % M;
-            const int iw = i <= a.nx / 2 ? i : a.nx - i;
-            sig = a.shift[iw * M + m];
-            col = (long long)fc * K + (long long)e * (N - 1) + (plane ? a.plane_off : 0);
-        }
-        row_sig[row] = sig;
-        row_col[row] = col;
-        if constexpr (ADJ) {
-            row_u0[row] = col >= 0 ? a.spec[col] : 0.0;
-            row_uN[row] = col >= 0 ? a.spec[col + N - 1] : 0.0;
-        }
-    }
-    __syncthreads();
-    // ---- stage F^T: tile[row][k] = f interior of the row's element ----
-    for (int idx = tid; idx < RT * NP; idx += 256) {
-        const int row = idx / NP, k = idx % NP;
-        const long long col = row_col[row];
-        double v = 0.0;
-        if (col >= 0 && k < n) v = a.spec[col + 1 + k];
-        tile[row * LD + k] = v;
-    }
-    __syncthreads();
-
-    // ---- Y^T = F^T Vinv^T ----
-    d4 acc[RBK][JT];
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
-    const int li = lane % 16, lk = lane / 16;
-    for (int ks = 0; ks < NP / 4; ++ks) {
-        const int kk = 4 * ks + lk;
-        double bv[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VinvT[(long long)kk * NP + 16 * (w * JT + jt) + li];
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + kk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
-        }
-    }
-    __syncthreads();  // every wave has read the F tile
-
-    // ---- shift, interface sweep, assemble Z^T in the tile ----
-    // q = y / (mu - sigma) stays in registers; GENERAL needs r0|rN . q per
-    // row (end-point derivatives of the particular solution), ADJ needs
-    // w0|wN . q (the adjoint load on zl, zr).  Each wave reduces its columns
-    // over the 16 lanes of a row, then the four waves' partials are summed.
-    double inv[RBK][JT][4];
-    double p0[RBK][4], pN[RBK][4];
-    const double *dv0 = ADJ ? a.w0 : a.r0, *dvN = ADJ ? a.wN : a.rN;
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) p0[rb][r] = pN[rb][r] = 0.0;
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) {
-            const int j = 16 * (w * JT + jt) + li;
-            const double mu = j < n ? a.mu[j] : 0.0;
-            const double v0 = (GENERAL || ADJ) && j < n ? dv0[j] : 0.0;
-            const double vN = (GENERAL || ADJ) && j < n ? dvN[j] : 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
-                const double d = mu - row_sig[row];
-                const double iv = (j < n) ? 1.0 / d : 0.0;
-                inv[rb][jt][r] = iv;
-                if (GENERAL || ADJ) {
-                    const double q = acc[rb][jt][r] * iv;
-                    p0[rb][r] = fma(v0, q, p0[rb][r]);
-                    pN[rb][r] = fma(vN, q, pN[rb][r]);
-                }
-            }
-        }
-    if constexpr (GENERAL || ADJ) {
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-#pragma unroll
-                for (int o = 1; o < 16; o <<= 1) {
-                    p0[rb][r] += __shfl_xor(p0[rb][r], o);
-                    pN[rb][r] += __shfl_xor(pN[rb][r], o);
-                }
-                if (li == 0) {
-                    const int row = 16 * rb + 4 * r + lk;
-                    part0[w][row] = p0[rb][r];
-                    partN[w][row] = pN[rb][r];
-                }
-            }
-        __syncthreads();
-        for (int row = tid; row < RT; row += 256) {
-            double s0 = 0.0, sN = 0.0;
-#pragma unroll
-            for (int ww = 0; ww < 4; ++ww) {
-                s0 += part0[ww][row];
-                sN += partN[ww][row];
-            }
-            row_d0[row] = s0;
-            row_dN[row] = sN;
-        }
-    }
-    if constexpr (ADJ) {
-        __syncthreads();
-        // transposed sweep (one element): zbar -> gbar, P0bar, PNbar
-        for (int row = tid; row < RT; row += 256) {
-            const int fl = row >> 1, plane = row & 1;
-            const int fc = fc0 + fl;
-            double P0b = 0.0, PNb = 0.0;
-            if (fl < fc_per_tile && fc < nfc) {
-                const int i = fc / M, m = fc % M;
-                const int iw = i <= a.nx / 2 ? i : a.nx - i;
-                const int mode = iw * M + m;
-                const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
-                const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
-                const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
-                const double lb = a.alpha_r * LN;
-                const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
-                const double zlb = row_u0[row] + row_d0[row];
-                const double zrb = row_uN[row] + row_dN[row] - (c / b) * zlb;
-                const double rhsrb = zrb / br;
-                const double rhsb = zlb / b - (lb / b) * rhsrb;
-                P0b = a.alpha_l * rhsb;
-                PNb = -a.alpha_r * rhsrb;
-                double *gre = plane ? a.gbar_im : a.gbar_re;
-                gre[fc] = rhsb;
-                gre[(long long)nfc + fc] = rhsrb;
-            }
-            row_zl[row] = P0b;  // reused: P0bar, PNbar per row
-            row_zr[row] = PNb;
-        }
-    } else if (GENERAL) {
-        __syncthreads();
-        // sweep over the elements of each real spectral column
-        for (int sc = tid; sc < 2 * fc_per_tile; sc += 256) {
-            const int fl = sc >> 1, plane = sc & 1;
-            const int fc = fc0 + fl;
-            if (fc >= nfc) continue;
-            const int i = fc / M, m = fc % M;
-            const int iw = i <= a.nx / 2 ? i : a.nx - i;
-            const int mode = iw * M + m;
-            const double *bsrc = plane ? a.bc_im : a.bc_re;
-            const double gl = bsrc[fc], gr = bsrc[(long long)nfc + fc];
-            const double d0ul = a.hom[4 * mode + 0], dNul = a.hom[4 * mode + 1];
-            const double d0ur = a.hom[4 * mode + 2], dNur = a.hom[4 * mode + 3];
-            const int r0w = (fl * 2 + plane) * E;
-            double bprev = -a.alpha_l * d0ul + a.beta_l;
-            double cprev = -a.alpha_l * d0ur;
-            double rprev = gl + a.alpha_l * row_d0[r0w];
-            row_zl[r0w] = bprev;  // b, rhs of the forward elimination kept in zl/zr
-            row_zr[r0w] = rprev;
-            for (int e = 1; e < E; ++e) {
-                const double lb = dNul;
-                const double be = dNur - d0ul - (cprev / bprev) * lb;
-                const double re = row_d0[r0w + e] - row_dN[r0w + e - 1] - (rprev / bprev) * lb;
-                bprev = be;
-                cprev = -d0ur;
-                rprev = re;
-                row_zl[r0w + e] = be;
-                row_zr[r0w + e] = re;
-            }
-            const double lb = a.alpha_r * dNul;
-            const double br = a.alpha_r * dNur + a.beta_r - (cprev / bprev) * lb;
-            const double rhsr = gr - a.alpha_r * row_dN[r0w + E - 1] - (rprev / bprev) * lb;
-            double zr = rhsr / br;
-            for (int e = E - 1; e >= 0; --e) {
-                const double be = row_zl[r0w + e], re = row_zr[r0w + e];
-                const double ce = (e == 0) ? -a.alpha_l * d0ur : -d0ur;
-                const double zl = (re - ce * zr) / be;
-                row_zl[r0w + e] = zl;
-                row_zr[r0w + e] = zr;
-                zr = zl;
-            }
-        }
-    } else if constexpr (!ADJ) {
-        // one element, Dirichlet data on both faces: zl = g_l, zr = g_r
-        for (int row = tid; row < RT; row += 256) {
-            const int fl = row >> 1, plane = row & 1;
-            const int fc = fc0 + fl;
-            double zl = 0.0, zr = 0.0;
-            if (fl < fc_per_tile && fc < nfc) {
-                const double *bsrc = plane ? a.bc_im : a.bc_re;
-                zl = bsrc[fc] / a.beta_l;
-                zr = bsrc[(long long)nfc + fc] / a.beta_r;
-            }
-            row_zl[row] = zl;
-            row_zr[row] = zr;
-        }
-    }
-    __syncthreads();
-    const double *zv0 = ADJ ? a.r0 : a.w0, *zvN = ADJ ? a.rN : a.wN;
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) {
-            const int j = 16 * (w * JT + jt) + li;
-            const double w0 = j < n ? zv0[j] : 0.0, wN = j < n ? zvN[j] : 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
-                const double h = fma(row_zl[row], w0, row_zr[row] * wN);
-                tile[row * LD + j] = (acc[rb][jt][r] + h) * inv[rb][jt][r];
-            }
-        }
-    __syncthreads();
-
-    // ---- U^T = Z^T V^T, stored along z ----
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int ks = 0; ks < NP / 4; ++ks) {
-        const int kk = 4 * ks + lk;
-        double bv[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VT[(long long)kk * NP + 16 * (w * JT + jt) + li];
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + kk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) {
-            const int k = 16 * (w * JT + jt) + li;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
-                const long long col = row_col[row];
-                if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
-            }
-        }
-    // element end values (the adjoint load has none: f is not used there)
-    for (int row = tid; row < RT; row += 256) {
-        const long long col = row_col[row];
-        if (col >= 0) {
-            a.spec[col] = ADJ ? 0.0 : row_zl[row];
-            a.spec[col + N - 1] = ADJ ? 0.0 : row_zr[row];
-        }
-    }
+    __shared__ DiagShared<NJ, RT, GENERAL, ADJ> sh;
+    diag_tile<NJ, RT, GENERAL, ADJ>(a, blockIdx.x, sh);
 }
 
 template <int NJ, int RT>

@@ -137,6 +137,19 @@ int launch_dot(const double *a, const double *b, long long n, double *partial, d
 
 }  // namespace hz
 
+struct hz_plan;
+namespace hz {
+// The four line-transform passes of a solve on `p`'s grid:
+//   0 R2C along y (in -> re/im; gl, gr -> boundary spectra when given),
+//   1 C2C along x forward (re/im and, with gl/gr, the boundary spectra),
+//   2 C2C along x inverse (boundary spectra too when gl_out/gr_out given),
+//   3 C2R along y (re/im -> out; boundary spectra -> gl_out, gr_out).
+FftArgs xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, double *im, double *out,
+                     int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out);
+// Arguments of the forward diagonalised mode solve on `p`'s workspace.
+DiagArgs diag_args(const hz_plan *p);
+}  // namespace hz
+
 // ---- the plan ----------------------------------------------------------------
 struct hz_plan {
     hz_plan_desc d{};

@@ -8,9 +8,11 @@
 //   spec --C2C^-1 along x--> spec --C2R along y--> u
 #include "hz_internal.h"
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 namespace hz {
@@ -229,7 +231,23 @@ int setup_diag(hz_plan *p)
     std::vector<double> A((size_t)n * n);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = p->Dzz[(size_t)(i + 1) * N + (j + 1)];
-    int st = eigen_decompose(A, n, p->mu, p->V, p->Vinv);
+    // The decomposition depends only on the interior operator: plans with
+    // the same (N, ne, Lz) share it (many-problem batches repeat sizes).
+    static std::mutex cache_mu;
+    static std::map<std::vector<double>, std::array<std::vector<double>, 3>> cache;
+    int st = HZ_OK;
+    {
+        std::lock_guard<std::mutex> lk(cache_mu);
+        auto it = cache.find(A);
+        if (it != cache.end()) {
+            p->mu = it->second[0];
+            p->V = it->second[1];
+            p->Vinv = it->second[2];
+        } else {
+            st = eigen_decompose(A, n, p->mu, p->V, p->Vinv);
+            if (!st && cache.size() < 256) cache[A] = {p->mu, p->V, p->Vinv};
+        }
+    }
     if (st) return set_error(st, "eigendecomposition of the GLL operator did not converge");
     p->w0.assign(n, 0.0);
     p->wN.assign(n, 0.0);
@@ -266,119 +284,114 @@ int setup_diag(hz_plan *p)
     return HZ_OK;
 }
 
+}  // namespace
+
+// Arguments of the four line-transform passes of a solve (see hz_internal.h).
+FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, double *im, double *out,
+                         int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out)
+{
+    const long long side = (long long)p->nx * p->M;
+    FftArgs a{};
+    a.lanes = lanes;
+    if (pass == 0) {  // R2C along y: [nx][ny][lanes] -> [nx][M][lanes]
+        a.in_re = in;
+        a.out_re = re;
+        a.out_im = im;
+        a.in_batch_stride = (long long)p->ny * lanes;
+        a.in_point_stride = lanes;
+        a.out_batch_stride = (long long)p->M * lanes;
+        a.out_point_stride = lanes;
+        a.nbatch = p->nx;
+        a.tw = p->d_twy;
+        if (gl && gr) {
+            a.xlanes = 2;
+            a.xin_re[0] = gl;
+            a.xin_re[1] = gr;
+            a.xout_re[0] = p->bc.re;
+            a.xout_re[1] = p->bc.re + side;
+            a.xout_im[0] = p->bc.im;
+            a.xout_im[1] = p->bc.im + side;
+            a.x_in_batch_stride = p->ny;
+            a.x_in_point_stride = 1;
+            a.x_out_batch_stride = p->M;
+            a.x_out_point_stride = 1;
+        }
+    } else if (pass == 1 || pass == 2) {  // C2C along x, in place
+        a.in_re = re;
+        a.in_im = im;
+        a.out_re = re;
+        a.out_im = im;
+        a.in_batch_stride = lanes;
+        a.in_point_stride = (long long)p->M * lanes;
+        a.out_batch_stride = lanes;
+        a.out_point_stride = (long long)p->M * lanes;
+        a.nbatch = p->M;
+        a.tw = p->d_twx;
+        if ((pass == 1 && gl && gr) || (pass == 2 && gl_out && gr_out)) {
+            a.xlanes = 2;
+            for (int sx = 0; sx < 2; ++sx) {
+                a.xin_re[sx] = a.xout_re[sx] = p->bc.re + sx * side;
+                a.xin_im[sx] = a.xout_im[sx] = p->bc.im + sx * side;
+            }
+            a.x_in_batch_stride = a.x_out_batch_stride = 1;
+            a.x_in_point_stride = a.x_out_point_stride = p->M;
+        }
+    } else {  // C2R along y: [nx][M][lanes] -> [nx][ny][lanes]
+        a.in_re = re;
+        a.in_im = im;
+        a.out_re = out;
+        a.in_batch_stride = (long long)p->M * lanes;
+        a.in_point_stride = lanes;
+        a.out_batch_stride = (long long)p->ny * lanes;
+        a.out_point_stride = lanes;
+        a.nbatch = p->nx;
+        a.tw = p->d_twy;
+        if (gl_out && gr_out) {
+            a.xlanes = 2;
+            for (int sx = 0; sx < 2; ++sx) {
+                a.xin_re[sx] = p->bc.re + sx * side;
+                a.xin_im[sx] = p->bc.im + sx * side;
+            }
+            a.xout_re[0] = gl_out;
+            a.xout_re[1] = gr_out;
+            a.x_in_batch_stride = p->M;
+            a.x_in_point_stride = 1;
+            a.x_out_batch_stride = p->ny;
+            a.x_out_point_stride = 1;
+        }
+    }
+    return a;
+}
+
+namespace {
+
 // Forward x-y transform of a [nx][ny][lanes] field into split half spectra;
 // with gl/gr the two face planes ride along as extra lanes of the same
 // launches and land in the boundary spectra.
 int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s,
                const double *gl = nullptr, const double *gr = nullptr)
 {
-    const long long side = (long long)p->nx * p->M;
-    FftArgs a{};
-    if (gl && gr) {
-        a.xlanes = 2;
-        a.xin_re[0] = gl;
-        a.xin_re[1] = gr;
-        a.xout_re[0] = p->bc.re;
-        a.xout_re[1] = p->bc.re + side;
-        a.xout_im[0] = p->bc.im;
-        a.xout_im[1] = p->bc.im + side;
-        a.x_in_batch_stride = p->ny;
-        a.x_in_point_stride = 1;
-        a.x_out_batch_stride = p->M;
-        a.x_out_point_stride = 1;
-    }
-    a.in_re = in;
-    a.out_re = re;
-    a.out_im = im;
-    a.in_batch_stride = (long long)p->ny * lanes;
-    a.in_point_stride = lanes;
-    a.out_batch_stride = (long long)p->M * lanes;
-    a.out_point_stride = lanes;
-    a.lanes = lanes;
-    a.nbatch = p->nx;
-    a.tw = p->d_twy;
-    int st = launch_fft(p->ny, FFT_R2C, a, s);
+    int st = launch_fft(p->ny, FFT_R2C, xy_pass_args(p, 0, in, re, im, nullptr, lanes, gl, gr, nullptr, nullptr), s);
     if (st) return st;
-    FftArgs b{};
-    b.in_re = re;
-    b.in_im = im;
-    b.out_re = re;
-    b.out_im = im;
-    b.in_batch_stride = lanes;
-    b.in_point_stride = (long long)p->M * lanes;
-    b.out_batch_stride = lanes;
-    b.out_point_stride = (long long)p->M * lanes;
-    b.lanes = lanes;
-    b.nbatch = p->M;
-    b.tw = p->d_twx;
-    if (gl && gr) {
-        b.xlanes = 2;
-        for (int sx = 0; sx < 2; ++sx) {
-            b.xin_re[sx] = b.xout_re[sx] = p->bc.re + sx * side;
-            b.xin_im[sx] = b.xout_im[sx] = p->bc.im + sx * side;
-        }
-        b.x_in_batch_stride = b.x_out_batch_stride = 1;
-        b.x_in_point_stride = b.x_out_point_stride = p->M;
-    }
-    return launch_fft(p->nx, FFT_C2C_FWD, b, s);
+    return launch_fft(p->nx, FFT_C2C_FWD, xy_pass_args(p, 1, nullptr, re, im, nullptr, lanes, gl, gr, nullptr, nullptr), s);
 }
 
+// Inverse transform; with gl_out/gr_out the boundary spectra are inverted
+// into them in the same launches.
 int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipStream_t s,
                double *gl_out = nullptr, double *gr_out = nullptr)
 {
-    const long long side = (long long)p->nx * p->M;
-    FftArgs b{};
-    if (gl_out && gr_out) {
-        b.xlanes = 2;
-        for (int sx = 0; sx < 2; ++sx) {
-            b.xin_re[sx] = b.xout_re[sx] = p->bc.re + sx * side;
-            b.xin_im[sx] = b.xout_im[sx] = p->bc.im + sx * side;
-        }
-        b.x_in_batch_stride = b.x_out_batch_stride = 1;
-        b.x_in_point_stride = b.x_out_point_stride = p->M;
-    }
-    b.in_re = re;
-    b.in_im = im;
-    b.out_re = re;
-    b.out_im = im;
-    b.in_batch_stride = lanes;
-    b.in_point_stride = (long long)p->M * lanes;
-    b.out_batch_stride = lanes;
-    b.out_point_stride = (long long)p->M * lanes;
-    b.lanes = lanes;
-    b.nbatch = p->M;
-    b.tw = p->d_twx;
-    int st = launch_fft(p->nx, FFT_C2C_INV, b, s);
+    int st = launch_fft(p->nx, FFT_C2C_INV,
+                        xy_pass_args(p, 2, nullptr, re, im, nullptr, lanes, nullptr, nullptr, gl_out, gr_out), s);
     if (st) return st;
-    FftArgs a{};
-    a.in_re = re;
-    a.in_im = im;
-    a.out_re = out;
-    a.in_batch_stride = (long long)p->M * lanes;
-    a.in_point_stride = lanes;
-    a.out_batch_stride = (long long)p->ny * lanes;
-    a.out_point_stride = lanes;
-    a.lanes = lanes;
-    a.nbatch = p->nx;
-    a.tw = p->d_twy;
-    if (gl_out && gr_out) {
-        a.xlanes = 2;
-        for (int sx = 0; sx < 2; ++sx) {
-            a.xin_re[sx] = p->bc.re + sx * side;
-            a.xin_im[sx] = p->bc.im + sx * side;
-        }
-        a.xout_re[0] = gl_out;
-        a.xout_re[1] = gr_out;
-        a.x_in_batch_stride = p->M;
-        a.x_in_point_stride = 1;
-        a.x_out_batch_stride = p->ny;
-        a.x_out_point_stride = 1;
-    }
-    return launch_fft(p->ny, FFT_C2R, a, s);
+    return launch_fft(p->ny, FFT_C2R,
+                      xy_pass_args(p, 3, nullptr, re, im, out, lanes, nullptr, nullptr, gl_out, gr_out), s);
 }
 
+}  // namespace
+
 // Kernel arguments of the z-diagonalised mode solve (forward map).
-DiagArgs diag_args(hz_plan *p)
+DiagArgs hz::diag_args(const hz_plan *p)
 {
     DiagArgs a{};
     a.nx = p->nx;
@@ -406,6 +419,8 @@ DiagArgs diag_args(hz_plan *p)
     a.beta_r = p->d.beta_r;
     return a;
 }
+
+namespace {
 
 // Records stage boundaries of one solve when profiling is on.
 struct StageRecorder {

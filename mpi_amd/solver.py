@@ -172,6 +172,62 @@ class HelmholtzPlan:
             pass
 
 
+class BatchSolver:
+    """Many DIAG-form plans solved together (hz_batch_*): one persistent
+    fused launch or one launch per stage, chosen by the autotuner on the first
+    solve and replayed from a HIP graph afterwards."""
+
+    NUM_VARIANTS = 6
+
+    def __init__(self, plans):
+        self.lib = load()
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
+        self.handle = ctypes.c_void_p()
+        check(self.lib.hz_batch_create(arr, len(self.plans), ctypes.byref(self.handle)))
+
+    def solve(self, fs, gls, grs, us=None, stream=None):
+        n = len(self.plans)
+        if us is None:
+            us = [p.empty_field() for p in self.plans]
+        for p, f, gl, gr, u in zip(self.plans, fs, gls, grs, us):
+            p._check_field(f, (p.nx, p.ny, p.nz))
+            p._check_field(gl, (p.nx, p.ny))
+            p._check_field(gr, (p.nx, p.ny))
+            p._check_field(u, (p.nx, p.ny, p.nz))
+        P = ctypes.c_void_p * n
+        check(self.lib.hz_batch_solve(self.handle, P(*[f.data_ptr() for f in fs]), P(*[g.data_ptr() for g in gls]),
+                                      P(*[g.data_ptr() for g in grs]), P(*[u.data_ptr() for u in us]),
+                                      _stream_ptr(stream)))
+        return us
+
+    def status(self):
+        check(self.lib.hz_batch_status(self.handle))
+
+    def force_variant(self, v):
+        check(self.lib.hz_batch_force_variant(self.handle, int(v)))
+
+    def tuning(self):
+        v, fused, rt, wgs = (ctypes.c_int(-1) for _ in range(4))
+        ms = np.zeros(self.NUM_VARIANTS)
+        check(self.lib.hz_batch_tuning(self.handle, ctypes.byref(v), ctypes.byref(fused), ctypes.byref(rt),
+                                       ctypes.byref(wgs), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       self.NUM_VARIANTS))
+        return {"variant": v.value, "fused": bool(fused.value), "tile_rows": rt.value,
+                "workgroups_per_cu": wgs.value, "variant_ms": ms.tolist()}
+
+    def close(self):
+        if self.handle:
+            self.lib.hz_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def lu_factor_batched(A, stream=None):
     """Reference LU (LU_Decomposition.cpp:7-72) of A[count, n, n] in place on the GPU.
     Returns (pivots[count, n-1] int32, det[count], status[count])."""

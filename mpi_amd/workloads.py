@@ -114,3 +114,46 @@ def random_problem(x, y, zeta, seed, nmodes, kmax, lmax, bc_modes=8, bc_kmax=8, 
     gl = random_modes_face(x, y, rng, bc_modes, bc_kmax, bc_amp)
     gr = random_modes_face(x, y, rng, bc_modes, bc_kmax, bc_amp)
     return f, gl, gr
+
+
+def batch_specs(count=512, seed=3):
+    """Sizes and shifts of the many-problem workload: nx, ny uniform in
+    {32, 64, 128}, N uniform over the odd values 17..129 (one element),
+    lambda log-uniform in [1e-2, 1e3]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        nx = (32, 64, 128)[rng.randint(0, 2)]
+        ny = (32, 64, 128)[rng.randint(0, 2)]
+        N = 17 + 2 * rng.randint(0, 56)
+        lam = 10.0 ** (-2.0 + 5.0 * rng.uniform())
+        out.append({"nx": nx, "ny": ny, "N": N, "lam": lam, "seed": rng.next_u64() & 0xFFFFFFFF})
+    return out
+
+
+def batch_fields(spec, zeta, nmodes=16, bc_modes=8):
+    """f = `nmodes` random modes cos(a x + b y + phi) P_l(zeta) with
+    a <= nx/2 - 1, b <= ny/2 - 1, l <= N - 1; Dirichlet faces from
+    `bc_modes` smooth modes of amplitude N(0, 0.1)."""
+    nx, ny, N = spec["nx"], spec["ny"], spec["N"]
+    x = np.arange(nx) * (2 * math.pi / nx)
+    y = np.arange(ny) * (2 * math.pi / ny)
+    rng = SplitMix64(spec["seed"])
+    a = np.array([rng.randint(0, nx // 2 - 1) for _ in range(nmodes)], dtype=np.float64)
+    b = np.array([rng.randint(0, ny // 2 - 1) for _ in range(nmodes)], dtype=np.float64)
+    l = np.array([rng.randint(0, N - 1) for _ in range(nmodes)])
+    c = np.array([rng.normal() for _ in range(nmodes)])
+    ph = np.array([2 * math.pi * rng.uniform() for _ in range(nmodes)])
+    P = legendre_table(np.asarray(zeta), N - 1)[l]
+    arg = a[:, None, None] * x[None, :, None] + b[:, None, None] * y[None, None, :] + ph[:, None, None]
+    f = np.einsum("mij,mk->ijk", c[:, None, None] * np.cos(arg), P, optimize=True)
+    kx, ky = min(8, nx // 2 - 1), min(8, ny // 2 - 1)
+    faces = []
+    for _ in range(2):
+        p = np.array([rng.randint(0, kx) for _ in range(bc_modes)], dtype=np.float64)
+        q = np.array([rng.randint(0, ky) for _ in range(bc_modes)], dtype=np.float64)
+        d = np.array([0.1 * rng.normal() for _ in range(bc_modes)])
+        ps = np.array([2 * math.pi * rng.uniform() for _ in range(bc_modes)])
+        arg = p[:, None, None] * x[None, :, None] + q[:, None, None] * y[None, None, :] + ps[:, None, None]
+        faces.append(np.einsum("m,mij->ij", d, np.cos(arg)))
+    return np.ascontiguousarray(f), faces[0], faces[1]

@@ -1,0 +1,157 @@
+"""Many problems of uneven size in one batch (hz_batch_*).
+
+Every enabled variant (per-stage launches, both tile heights; the fused
+persistent variants are switched off) must reproduce each problem's
+single-plan DIAG solve, which is
+itself pinned to the oracle; a few problems are also checked against the
+oracle directly.  Bar: relative max-norm difference <= 1e-13 against the
+single-plan solve, <= 1e-10 against the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+# (nx, ny, N, ne, lam, robin)
+MIXED_SET = [(16, 32, 17, 1, 0.5, False), (32, 16, 33, 2, 3.0, True), (64, 64, 65, 1, 10.0, False),
+             (128, 32, 129, 1, 1e-2, False), (32, 128, 97, 1, 1e3, True), (256, 16, 21, 3, 2.0, False),
+             (16, 256, 9, 4, 7.0, True)]
+
+
+def make_set(spec):
+    from mpi_amd.solver import HelmholtzPlan
+    plans, inputs, descs = [], [], []
+    for k, (nx, ny, N, ne, lam, robin) in enumerate(spec):
+        al, bl, ar, br = (0.7, 1.3, 0.4, 2.0) if robin else (0.0, 1.0, 0.0, 1.0)
+        p = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam,
+                          alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form="diag")
+        rng = np.random.default_rng(100 + k)
+        f = rng.standard_normal((nx, ny, p.nz))
+        gl = rng.standard_normal((nx, ny))
+        gr = rng.standard_normal((nx, ny))
+        plans.append(p)
+        inputs.append((f, gl, gr))
+        descs.append(po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2,
+                                  alphal=al, betal=bl, alphar=ar, betar=br, lam=lam))
+    return plans, inputs, descs
+
+
+FUSED_OFF = ("fused persistent variant switched off: its first GPU run did not finish (dependency waits "
+             "ran to the spin limit); cause not established")
+
+
+@pytest.mark.parametrize("variant", [pytest.param(v, marks=pytest.mark.skip(reason=FUSED_OFF)) for v in range(4)]
+                         + [4, 5])
+def test_batch_variant_matches_single_solves(cuda, variant):
+    from mpi_amd.solver import BatchSolver
+    plans, inputs, _ = make_set(MIXED_SET)
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [dev(g) for _, g, _ in inputs]
+    grs = [dev(g) for _, _, g in inputs]
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    b = BatchSolver(plans)
+    b.force_variant(variant)
+    us = b.solve(fs, gls, grs)
+    b.status()
+    for k, (u, w) in enumerate(zip(us, want)):
+        e = rel(u.cpu().numpy(), w)
+        assert e <= 1e-13, f"problem {k} ({MIXED_SET[k]}): {e:.3e}"
+    b.close()
+
+
+def test_batch_against_oracle(cuda):
+    from mpi_amd.solver import BatchSolver
+    plans, inputs, descs = make_set(MIXED_SET)
+    b = BatchSolver(plans)
+    us = b.solve([dev(f) for f, _, _ in inputs], [dev(g) for _, g, _ in inputs], [dev(g) for _, _, g in inputs])
+    b.status()
+    print("tuning:", b.tuning())
+    for k in (0, 1, 4, 6):
+        f, gl, gr = inputs[k]
+        e = rel(us[k].cpu().numpy(), po.solve(descs[k], f, gl, gr))
+        assert e <= 1e-10, f"problem {k}: {e:.3e}"
+    b.close()
+
+
+def test_batch_graph_replay_and_new_buffers(cuda):
+    """Replays of the recorded graph, with unrelated GPU work and host
+    synchronisation between them, reproduce the first result exactly; new
+    output buffers are picked up."""
+    import torch
+    from mpi_amd.solver import BatchSolver
+    plans, inputs, _ = make_set(MIXED_SET[:4])
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [dev(g) for _, g, _ in inputs]
+    grs = [dev(g) for _, _, g in inputs]
+    b = BatchSolver(plans)
+    first = [u.clone() for u in b.solve(fs, gls, grs)]
+    for rep in range(3):
+        junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+        junk.mul_(2.0).sum().item()
+        torch.cuda.synchronize()
+        us = b.solve(fs, gls, grs)
+        torch.cuda.synchronize()
+        for u, w in zip(us, first):
+            assert torch.equal(u, w), f"replay {rep} differs"
+    # same shapes, fresh input values in fresh buffers
+    fs2 = [f * 2.0 for f in fs]
+    us2 = b.solve(fs2, gls, grs)
+    for p, u, f, gl, gr in zip(plans, us2, fs2, gls, grs):
+        w = p.solve(f, gl, gr)
+        assert rel(u.cpu().numpy(), w.cpu().numpy()) <= 1e-13
+    b.close()
+
+
+def test_batch_many_problem_workload(cuda):
+    """512 problems: nx, ny in {32, 64, 128}, N odd in [17, 129], lambda
+    log-uniform in [1e-2, 1e3]; every one checked against its single solve."""
+    from mpi_amd import workloads as wl
+    from mpi_amd.solver import BatchSolver, HelmholtzPlan
+    specs = wl.batch_specs(512, seed=3)
+    plans, fs, gls, grs = [], [], [], []
+    for s in specs:
+        p = HelmholtzPlan(s["nx"], s["ny"], s["N"], 1, lam=s["lam"], form="diag")
+        f, gl, gr = wl.batch_fields(s, p.z_nodes() - 1.0)
+        plans.append(p)
+        fs.append(dev(f))
+        gls.append(dev(gl))
+        grs.append(dev(gr))
+    b = BatchSolver(plans)
+    us = b.solve(fs, gls, grs)
+    b.status()
+    print("tuning:", b.tuning())
+    worst = 0.0
+    for p, u, f, gl, gr in zip(plans, us, fs, gls, grs):
+        worst = max(worst, rel(u.cpu().numpy(), p.solve(f, gl, gr).cpu().numpy()))
+    print(f"worst relative difference over 512 problems: {worst:.3e}")
+    assert worst <= 1e-13
+    b.close()
+
+
+def test_batch_rejects_bad_sets(cuda):
+    from mpi_amd._lib import HzError
+    from mpi_amd.solver import BatchSolver, HelmholtzPlan
+    p = HelmholtzPlan(16, 16, 9, 1, form="diag")
+    q = HelmholtzPlan(16, 16, 9, 1, form="plain")
+    with pytest.raises(HzError):
+        BatchSolver([p, p])
+    with pytest.raises(HzError):
+        BatchSolver([p, q])
+    b = BatchSolver([p])
+    with pytest.raises(HzError):
+        b.force_variant(0)  # fused variants are switched off
+    b.close()
