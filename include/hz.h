@@ -79,7 +79,7 @@ typedef struct {
                                 < 0 = none, <= 16)                          */
     double mixed_fp64_cond;  /* MIXED only: shift sigma below which a mode keeps
                                 an FP64 operator (0 = built-in rule:
-                                cond * 2^-24 > 5e-2)                        */
+                                cond * 2^-24 > 1)                        */
 } hz_plan_desc;
 
 /* Replaces CreatePDEPlan (NonPeriodic3D_MPI.cpp:40-134): GLL nodes, scaled
@@ -195,6 +195,9 @@ int hz_batch_destroy(hz_batch *batch);
  * *sweeps = number of values written (refine_sweeps + 1, at most max_sweeps).
  * No counterpart in the reference (its LU is FP64 only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
+/* Plan-time autotuning result: tile rows of the diagonalised mode kernel
+ * (DIAG plans; 0 otherwise) and the measured ms of each candidate. */
+int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *choice_ms, int max_choices);
 /* Number of modes of a MIXED plan that keep an FP64 operator. */
 int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 

@@ -6,7 +6,7 @@ namespace hz {
 namespace {
 
 template <int NJ, int RT, bool GENERAL, bool ADJ>
-__global__ __launch_bounds__(256) void diag_modes_kernel(const DiagArgs a)
+__global__ __launch_bounds__(256, 2) void diag_modes_kernel(const DiagArgs a)
 {
     __shared__ DiagShared<NJ, RT, GENERAL, ADJ> sh;
     diag_tile<NJ, RT, GENERAL, ADJ>(a, blockIdx.x, sh);
@@ -43,12 +43,24 @@ int diag_padded(int n)
     return 0;
 }
 
+int diag_tile_choices(int NP, int *rt, int max)
+{
+    static const int c64[] = {32, 64}, c128[] = {32, 48, 64}, c256[] = {16, 32};
+    const int *c = NP == 64 ? c64 : NP == 128 ? c128 : c256;
+    const int k = NP == 128 ? 3 : 2;
+    for (int i = 0; i < k && i < max; ++i) rt[i] = c[i];
+    return k;
+}
+
 int launch_diag_modes(const DiagArgs &a, hipStream_t s)
 {
     switch (diag_padded(a.n)) {
-    case 64: return launch_t<4, 64>(a, s);
-    case 128: return launch_t<8, 64>(a, s);
-    case 256: return launch_t<16, 32>(a, s);
+    case 64:
+        return a.rt == 32 ? launch_t<4, 32>(a, s) : launch_t<4, 64>(a, s);
+    case 128:
+        return a.rt == 32 ? launch_t<8, 32>(a, s) : a.rt == 48 ? launch_t<8, 48>(a, s) : launch_t<8, 64>(a, s);
+    case 256:
+        return a.rt == 16 ? launch_t<16, 16>(a, s) : launch_t<16, 32>(a, s);
     default: return set_error(HZ_ERR_UNSUPPORTED, "DIAG form: N-2 must be <= 256");
     }
 }

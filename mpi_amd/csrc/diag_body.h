@@ -45,6 +45,57 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// acc[rb][jt] += A B over the k index, A = the LDS tile (rows 16 rb + i,
+// column k), B[k][j] = B[k * NP + j] from L2 (j = the wave's column tiles).
+// B is loaded two k-steps ahead so its L2 latency overlaps the MFMAs of the
+// steps in between.
+template <int NP, int LD, int RBK, int JT>
+__device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile, const double *__restrict__ B,
+                                          int w, int li, int lk)
+{
+    constexpr int NK = NP / 4;
+    static_assert(NK % 2 == 0, "two k-steps per iteration");
+    const double *bcol = B + 16 * (w * JT) + li + (long long)lk * NP;
+    double b0[JT], b1[JT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+        b0[jt] = bcol[16 * jt];
+        b1[jt] = bcol[(long long)4 * NP + 16 * jt];
+    }
+    for (int ks = 0; ks < NK; ks += 2) {
+        double n0[JT], n1[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) n0[jt] = n1[jt] = 0.0;
+        if (ks + 2 < NK) {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) n0[jt] = bcol[(long long)4 * (ks + 2) * NP + 16 * jt];
+        }
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb) {
+            const double av = tile[(16 * rb + li) * LD + 4 * ks + lk];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0[jt], acc[rb][jt], 0, 0, 0);
+        }
+        if (ks + 3 < NK) {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) n1[jt] = bcol[(long long)4 * (ks + 3) * NP + 16 * jt];
+        }
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb) {
+            const double av = tile[(16 * rb + li) * LD + 4 * (ks + 1) + lk];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1[jt], acc[rb][jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) {
+            b0[jt] = n0[jt];
+            b1[jt] = n1[jt];
+        }
+    }
+}
+
 // LDS of one diag tile.
 template <int NJ, int RT, bool GENERAL, bool ADJ>
 struct DiagShared {
@@ -123,19 +174,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
     const int li = lane % 16, lk = lane / 16;
-    for (int ks = 0; ks < NP / 4; ++ks) {
-        const int kk = 4 * ks + lk;
-        double bv[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VinvT[(long long)kk * NP + 16 * (w * JT + jt) + li];
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + kk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
-        }
-    }
+    tile_gemm<NP, LD, RBK, JT>(acc, tile, a.VinvT, w, li, lk);
     __syncthreads();  // every wave has read the F tile
 
     // ---- shift, interface sweep, assemble Z^T in the tile ----
@@ -308,19 +347,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     for (int rb = 0; rb < RBK; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int ks = 0; ks < NP / 4; ++ks) {
-        const int kk = 4 * ks + lk;
-        double bv[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = a.VT[(long long)kk * NP + 16 * (w * JT + jt) + li];
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + kk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[jt], acc[rb][jt], 0, 0, 0);
-        }
-    }
+    tile_gemm<NP, LD, RBK, JT>(acc, tile, a.VT, w, li, lk);
 #pragma unroll
     for (int rb = 0; rb < RBK; ++rb)
 #pragma unroll

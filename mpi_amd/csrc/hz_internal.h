@@ -91,8 +91,11 @@ struct DiagArgs {
     double alpha_l, beta_l, alpha_r, beta_r;
     int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
     double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
+    int rt;                     // tile rows (spectral columns x planes x elements); 0 = default
 };
 int diag_padded(int n);
+// Tile heights offered for padded size NP (autotuned per plan).
+int diag_tile_choices(int NP, int *rt, int max);
 int launch_diag_modes(const DiagArgs &a, hipStream_t s);
 
 // MIXED form (mixed.hip).
@@ -168,6 +171,8 @@ struct hz_plan {
     double *d_mu = nullptr, *d_w0 = nullptr, *d_wN = nullptr, *d_r0 = nullptr, *d_rN = nullptr;
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
+    int diag_rt = 0;                                  // autotuned tile rows
+    std::vector<double> diag_rt_ms;                   // tuning times per choice
     double *d_partial = nullptr;                      // reduction scratch
     // MIXED form
     int ldx = 0, n32 = 0, n64 = 0, sweeps = 0;

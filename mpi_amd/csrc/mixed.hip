@@ -22,8 +22,10 @@
 // every mode after the solve and after each sweep goes to `hist`.
 //
 // The FP32 operator is read from HBM once per solve and stays in registers
-// for all sweeps: thread (i, h) holds X[i][64h .. 64h+63].  FP64 operators
-// are streamed from memory at each application (few modes).
+// for all sweeps: thread (i, h) holds X[i][64h .. 64h+63].  The FP64
+// residual streams Dzz_int from L2 (holding it in registers as well costs
+// more in occupancy than it saves, measured).  FP64 operators are streamed
+// from memory at each application (few modes).
 #include "hz_internal.h"
 
 namespace hz {

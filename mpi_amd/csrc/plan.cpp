@@ -115,10 +115,11 @@ int compute_homogeneous(hz_plan *p)
 // MIXED form: which modes keep FP64 operators, and the operators themselves.
 // cond(Dzz_int - sigma I) <= cond(V) max|mu - sigma| / min|mu - sigma|; FP32
 // refinement contracts the error by at most about cond * 2^-24 per sweep
-// (the measured contraction is far smaller), so modes above kMixedRate keep
-// FP64 (or, if the caller gave one, modes whose shift is below
-// desc.mixed_fp64_cond).
-constexpr double kMixedRate = 5e-2;
+// (the measured contraction is far smaller: 3e-6 for modes where this bound
+// is 0.1), so modes where the bound exceeds kMixedRate -- FP32 refinement
+// may then stall -- keep FP64 (or, if the caller gave one, modes whose shift
+// is below desc.mixed_fp64_cond).
+constexpr double kMixedRate = 1.0;
 
 int mixed_operators(hz_plan *p)
 {
@@ -417,10 +418,51 @@ DiagArgs hz::diag_args(const hz_plan *p)
     a.beta_l = p->d.beta_l;
     a.alpha_r = p->d.alpha_r;
     a.beta_r = p->d.beta_r;
+    a.rt = p->diag_rt;
     return a;
 }
 
 namespace {
+
+// Plan-time autotuning of the diagonalised mode kernel's tile height: each
+// choice is timed (one warm-up, three timed launches) on the plan's own
+// (zeroed) spectrum and the fastest kept.
+int tune_diag(hz_plan *p)
+{
+    int choices[4];
+    const int k = diag_tile_choices(p->NP, choices, 4);
+    HZ_HIP(hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->M * p->K));
+    HZ_HIP(hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M));
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    HZ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HZ_HIP(hipEventCreate(&e0));
+    HZ_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int st = HZ_OK;
+    p->diag_rt_ms.assign(k, 0.0);
+    for (int i = 0; i < k && !st; ++i) {
+        if (choices[i] < 2 * p->ne) continue;  // a tile must hold one column's elements
+        DiagArgs a = diag_args(p);
+        a.rt = choices[i];
+        if ((st = launch_diag_modes(a, s))) break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3 && !st; ++r) st = launch_diag_modes(a, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        p->diag_rt_ms[i] = ms / 3.0;
+        if (ms < best) {
+            best = ms;
+            p->diag_rt = choices[i];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return st;
+}
 
 // Records stage boundaries of one solve when profiling is on.
 struct StageRecorder {
@@ -514,6 +556,10 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
+    if (d.form == HZ_FORM_DIAG && (st = tune_diag(p))) {
+        free_plan(p);
+        return st;
+    }
     *plan = p;
     return HZ_OK;
 }
@@ -764,6 +810,15 @@ int hz_mixed_history(const hz_plan *p, double *rel_residual, int max_sweeps, int
         rel_residual[s] = v;
     }
     *sweeps = std::min(S, max_sweeps);
+    return HZ_OK;
+}
+
+int hz_plan_tuning(const hz_plan *p, int *diag_tile_rows, double *choice_ms, int max_choices)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    if (diag_tile_rows) *diag_tile_rows = p->diag_rt;
+    if (choice_ms)
+        for (int i = 0; i < (int)p->diag_rt_ms.size() && i < max_choices; ++i) choice_ms[i] = p->diag_rt_ms[i];
     return HZ_OK;
 }
 

@@ -155,6 +155,14 @@ class HelmholtzPlan:
                                         max_sweeps, ctypes.byref(n)))
         return h[: n.value]
 
+    def tuning(self):
+        """Plan-time autotuning: diag tile rows and the ms of each candidate."""
+        rt = ctypes.c_int(0)
+        ms = np.zeros(4)
+        check(self.lib.hz_plan_tuning(self.handle, ctypes.byref(rt), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      4))
+        return {"diag_tile_rows": rt.value, "choice_ms": [v for v in ms.tolist() if v > 0]}
+
     def mixed_fp64_modes(self):
         k = ctypes.c_int(0)
         check(self.lib.hz_mixed_fp64_modes(self.handle, ctypes.byref(k)))

@@ -82,3 +82,21 @@ def test_diag_general_against_oracle(cuda, N, ne, robin):
     u = plan.solve(to_dev(f), to_dev(gl), to_dev(gr)).cpu().numpy()
     uo = po.solve(desc, f, gl, gr)
     assert rel_err(u, uo) <= TOL
+
+
+@pytest.mark.parametrize("N", [33, 129, 257])
+def test_diag_plan_autotune(cuda, N):
+    """The plan-time tuner picks one of its candidate tile heights, and every
+    candidate gives the same answer to round-off."""
+    from mpi_amd.solver import HelmholtzPlan
+    plan = HelmholtzPlan(32, 16, N, 1, lam=2.0, form="diag")
+    t = plan.tuning()
+    print(f"N={N}: {t}")
+    assert t["diag_tile_rows"] in {16, 32, 48, 64} and len(t["choice_ms"]) >= 2
+    desc = po.make_desc(32, 16, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1, lam=2.0)
+    rng = np.random.default_rng(N)
+    f = rng.standard_normal((32, 16, plan.nz))
+    g = rng.standard_normal((32, 16))
+    u = plan.solve(to_dev(f), to_dev(g), to_dev(g)).cpu().numpy()
+    assert rel_err(u, po.solve(desc, f, g, g)) <= TOL
+    plan.close()
