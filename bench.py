@@ -125,7 +125,7 @@ def time_form(form, f, gl, gr, steps, warmup):
     plan.status()
     err = reference_sample_error(u.cpu().numpy())
     res = {"form": form, "solves_per_s": steps / (t1 - t0), "ms_per_solve": 1e3 * (t1 - t0) / steps,
-           "stage_ms": stages, "rel_err_vs_reference": err, "profiled_solves": nrec}
+           "stage_ms": stages, "rel_err_vs_reference": err, "profiled_solves": nrec, "tuning": plan.tuning()}
     plan.close()
     return res
 
@@ -398,7 +398,8 @@ def main():
                                "Dirichlet z faces, 64 random modes", "nx": NX, "ny": NY, "n_gll": N,
                    "elements": 1, "lambda": LAM, "form": chosen["form"]},
         "forms": {k: {"solves_per_s": v["solves_per_s"], "ms_per_solve": v["ms_per_solve"],
-                      "rel_err_vs_reference": v["rel_err_vs_reference"], "stage_ms": v["stage_ms"]}
+                      "rel_err_vs_reference": v["rel_err_vs_reference"], "stage_ms": v["stage_ms"],
+                      "tuning": v["tuning"]}
                   for k, v in results.items()},
         "chosen_vs_plain": (chosen["solves_per_s"] / results["plain"]["solves_per_s"]) if "plain" in results else None,
         "roofline": roof,

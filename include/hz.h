@@ -195,9 +195,13 @@ int hz_batch_destroy(hz_batch *batch);
  * *sweeps = number of values written (refine_sweeps + 1, at most max_sweeps).
  * No counterpart in the reference (its LU is FP64 only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
-/* Plan-time autotuning result: tile rows of the diagonalised mode kernel
- * (DIAG plans; 0 otherwise) and the measured ms of each candidate. */
-int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *choice_ms, int max_choices);
+/* Plan-time autotuning results: tile rows of the diagonalised mode kernel
+ * (DIAG plans; 0 otherwise) with the measured ms of each candidate, and the
+ * z lanes per work-group chosen for each of the four transform passes
+ * (R2C y, C2C x, C2C^-1 x, C2R y; 0 = the length's only shape) with the
+ * winner's ms.  Any pointer may be NULL. */
+int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *diag_ms, int max_diag,
+                   int *fft_lanes, double *fft_ms);
 /* Number of modes of a MIXED plan that keep an FP64 operator. */
 int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 

@@ -64,7 +64,7 @@ SIGNATURES = {
     "hz_batch_force_variant": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_mixed_history": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI]),
     "hz_mixed_fp64_modes": (ctypes.c_int, [_VP, _PI]),
-    "hz_plan_tuning": (ctypes.c_int, [_VP, _PI, _PD, ctypes.c_int]),
+    "hz_plan_tuning": (ctypes.c_int, [_VP, _PI, _PD, ctypes.c_int, _PI, _PD]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),
     "hz_host_eigen": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD]),
     "hz_plan_profile": (ctypes.c_int, [_VP, ctypes.c_int]),

@@ -55,6 +55,14 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
 
 }  // namespace
 
+int fft_width_choices(int L, int *w, int max)
+{
+    if (L < 64 || L > 256) return 0;
+    static const int c[] = {8, 16, 32};
+    for (int i = 0; i < 3 && i < max; ++i) w[i] = c[i];
+    return 3;
+}
+
 int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
 {
     if (a.nbatch <= 0 || a.lanes + a.xlanes <= 0) return HZ_OK;
@@ -64,9 +72,15 @@ int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
     case 8: return launch_kind<8, 8, 1, 16>(kind, a, s);
     case 16: return launch_kind<16, 4, 4, 16>(kind, a, s);
     case 32: return launch_kind<32, 8, 4, 16>(kind, a, s);
-    case 64: return launch_kind<64, 8, 8, 16>(kind, a, s);
-    case 128: return launch_kind<128, 16, 8, 16>(kind, a, s);
-    case 256: return launch_kind<256, 16, 16, 16>(kind, a, s);
+    case 64:
+        return a.w == 8 ? launch_kind<64, 8, 8, 8>(kind, a, s)
+               : a.w == 32 ? launch_kind<64, 8, 8, 32>(kind, a, s) : launch_kind<64, 8, 8, 16>(kind, a, s);
+    case 128:
+        return a.w == 8 ? launch_kind<128, 16, 8, 8>(kind, a, s)
+               : a.w == 32 ? launch_kind<128, 16, 8, 32>(kind, a, s) : launch_kind<128, 16, 8, 16>(kind, a, s);
+    case 256:
+        return a.w == 8 ? launch_kind<256, 16, 16, 8>(kind, a, s)
+               : a.w == 32 ? launch_kind<256, 16, 16, 32>(kind, a, s) : launch_kind<256, 16, 16, 16>(kind, a, s);
     case 512: return launch_kind<512, 32, 16, 8>(kind, a, s);
     case 1024: return launch_kind<1024, 32, 32, 4>(kind, a, s);
     default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [2, 1024]");

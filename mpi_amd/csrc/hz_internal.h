@@ -59,8 +59,11 @@ struct FftArgs {
     const double *xin_re[2], *xin_im[2];
     double *xout_re[2], *xout_im[2];
     long long x_in_batch_stride, x_in_point_stride, x_out_batch_stride, x_out_point_stride;
+    int w;                         // z lanes per work-group (0 = default for L)
 };
 int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s);
+// Lane widths offered for length L (autotuned per plan); returns the count.
+int fft_width_choices(int L, int *w, int max);
 
 // Arguments of the plain-LU mode kernel (lu.hip).
 struct LuModeArgs {
@@ -172,6 +175,8 @@ struct hz_plan {
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
     int diag_rt = 0;                                  // autotuned tile rows
+    int fft_w[4] = {0, 0, 0, 0};                      // autotuned lanes per pass
+    double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass
     std::vector<double> diag_rt_ms;                   // tuning times per choice
     double *d_partial = nullptr;                      // reduction scratch
     // MIXED form

@@ -294,6 +294,7 @@ FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *r
     const long long side = (long long)p->nx * p->M;
     FftArgs a{};
     a.lanes = lanes;
+    a.w = p->fft_w[pass];
     if (pass == 0) {  // R2C along y: [nx][ny][lanes] -> [nx][M][lanes]
         a.in_re = in;
         a.out_re = re;
@@ -464,6 +465,59 @@ int tune_diag(hz_plan *p)
     return st;
 }
 
+// Plan-time autotuning of the transform passes' lane width: each choice of
+// each pass is timed (one warm-up, three timed launches) on scratch buffers
+// of the plan's shape and the fastest kept.
+int tune_fft(hz_plan *p)
+{
+    const size_t phys = (size_t)p->nx * p->ny * p->K;
+    double *in = nullptr;
+    if (hipMalloc((void **)&in, sizeof(double) * phys) != hipSuccess)
+        return set_error(HZ_ERR_ALLOC, "hipMalloc: tuning buffer");
+    int st = HZ_OK;
+    if (hipMemset(in, 0, sizeof(double) * phys) != hipSuccess ||
+        hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->M * p->K) != hipSuccess ||
+        hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M) != hipSuccess)
+        st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = set_error(HZ_ERR_HIP, "tuning stream failed");
+    static const FftKind kinds[4] = {FFT_R2C, FFT_C2C_FWD, FFT_C2C_INV, FFT_C2R};
+    for (int pass = 0; pass < 4 && !st; ++pass) {
+        const int L = (pass == 0 || pass == 3) ? p->ny : p->nx;
+        int w[4];
+        const int k = fft_width_choices(L, w, 4);
+        int bestw = 0;
+        double best = 1e300;
+        for (int i = 0; i < k && !st; ++i) {
+            p->fft_w[pass] = w[i];
+            // the faces ride along as in a solve (passes 0 and 1)
+            FftArgs a = xy_pass_args(p, pass, in, p->spec.re, p->spec.im, in, p->K, pass < 2 ? in : nullptr,
+                                     pass < 2 ? in : nullptr, nullptr, nullptr);
+            if ((st = launch_fft(L, kinds[pass], a, s))) break;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && !st; ++r) st = launch_fft(L, kinds[pass], a, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                bestw = w[i];
+            }
+        }
+        p->fft_w[pass] = bestw;
+        p->fft_w_ms[pass] = k ? best / 3.0 : 0.0;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(in);
+    return st;
+}
+
 // Records stage boundaries of one solve when profiling is on.
 struct StageRecorder {
     hz_plan *p;
@@ -556,7 +610,7 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
-    if (d.form == HZ_FORM_DIAG && (st = tune_diag(p))) {
+    if ((st = tune_fft(p)) || (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
         free_plan(p);
         return st;
     }
@@ -813,12 +867,17 @@ int hz_mixed_history(const hz_plan *p, double *rel_residual, int max_sweeps, int
     return HZ_OK;
 }
 
-int hz_plan_tuning(const hz_plan *p, int *diag_tile_rows, double *choice_ms, int max_choices)
+int hz_plan_tuning(const hz_plan *p, int *diag_tile_rows, double *diag_ms, int max_diag, int *fft_lanes,
+                   double *fft_ms)
 {
     if (!p) return set_error(HZ_ERR_ARG, "null argument");
     if (diag_tile_rows) *diag_tile_rows = p->diag_rt;
-    if (choice_ms)
-        for (int i = 0; i < (int)p->diag_rt_ms.size() && i < max_choices; ++i) choice_ms[i] = p->diag_rt_ms[i];
+    if (diag_ms)
+        for (int i = 0; i < (int)p->diag_rt_ms.size() && i < max_diag; ++i) diag_ms[i] = p->diag_rt_ms[i];
+    for (int pass = 0; pass < 4; ++pass) {
+        if (fft_lanes) fft_lanes[pass] = p->fft_w[pass];
+        if (fft_ms) fft_ms[pass] = p->fft_w_ms[pass];
+    }
     return HZ_OK;
 }
 

@@ -156,12 +156,17 @@ class HelmholtzPlan:
         return h[: n.value]
 
     def tuning(self):
-        """Plan-time autotuning: diag tile rows and the ms of each candidate."""
+        """Plan-time autotuning: diag tile rows (and the ms of each candidate),
+        z lanes per work-group of the four transform passes (and their ms)."""
+        P = ctypes.POINTER(ctypes.c_double)
         rt = ctypes.c_int(0)
         ms = np.zeros(4)
-        check(self.lib.hz_plan_tuning(self.handle, ctypes.byref(rt), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                      4))
-        return {"diag_tile_rows": rt.value, "choice_ms": [v for v in ms.tolist() if v > 0]}
+        lanes = (ctypes.c_int * 4)()
+        fms = np.zeros(4)
+        check(self.lib.hz_plan_tuning(self.handle, ctypes.byref(rt), ms.ctypes.data_as(P), 4, lanes,
+                                      fms.ctypes.data_as(P)))
+        return {"diag_tile_rows": rt.value, "choice_ms": [v for v in ms.tolist() if v > 0],
+                "fft_lanes": list(lanes), "fft_ms": fms.tolist()}
 
     def mixed_fp64_modes(self):
         k = ctypes.c_int(0)

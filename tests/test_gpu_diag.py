@@ -86,17 +86,20 @@ def test_diag_general_against_oracle(cuda, N, ne, robin):
 
 @pytest.mark.parametrize("N", [33, 129, 257])
 def test_diag_plan_autotune(cuda, N):
-    """The plan-time tuner picks one of its candidate tile heights, and every
-    candidate gives the same answer to round-off."""
+    """The plan-time tuner picks one of its candidate shapes, and the tuned
+    plan solves to the oracle's answer."""
     from mpi_amd.solver import HelmholtzPlan
-    plan = HelmholtzPlan(32, 16, N, 1, lam=2.0, form="diag")
+    plan = HelmholtzPlan(64, 16, N, 1, lam=2.0, form="diag")
     t = plan.tuning()
     print(f"N={N}: {t}")
     assert t["diag_tile_rows"] in {16, 32, 48, 64} and len(t["choice_ms"]) >= 2
-    desc = po.make_desc(32, 16, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1, lam=2.0)
+    # nx = 64 offers three lane widths (passes 1, 2), ny = 16 one shape (passes 0, 3)
+    assert t["fft_lanes"][0] == 0 and t["fft_lanes"][3] == 0
+    assert t["fft_lanes"][1] in {8, 16, 32} and t["fft_lanes"][2] in {8, 16, 32}
+    desc = po.make_desc(64, 16, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1, lam=2.0)
     rng = np.random.default_rng(N)
-    f = rng.standard_normal((32, 16, plan.nz))
-    g = rng.standard_normal((32, 16))
+    f = rng.standard_normal((64, 16, plan.nz))
+    g = rng.standard_normal((64, 16))
     u = plan.solve(to_dev(f), to_dev(g), to_dev(g)).cpu().numpy()
     assert rel_err(u, po.solve(desc, f, g, g)) <= TOL
     plan.close()
