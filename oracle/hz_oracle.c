@@ -738,6 +738,154 @@ done:
     return status;
 }
 
+/* Periodic3D_MPI.cpp:233-863 for one rank (size = 1): the same transforms,
+ * operators and fundamental solutions (:184-232), then the cyclic
+ * element-interface system.  With Ns = 1 (:323-333, :655-664) the single
+ * interface value is rhs0/b; with Ns > 1 a forward sweep carries two right-
+ * hand sides, the particular one (rhs0) and the periodic coupling column
+ * (rhs1) (:336-376), the backward sweep gives zeta and eta (:666-694), and the
+ * closure alpha = -(zl + zr)/(1 + el + er) of the last element combines them
+ * (:696-711).  Assembly and the inverse transform as in the non-periodic
+ * solver (:854-862).  desc->size must be 1; the Robin fields are unused. */
+int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u)
+{
+    const int Nx = d->Nx, Ny = d->Ny, N = d->N;
+    if (!is_pow2(Nx) || !is_pow2(Ny)) return HZO_ERR_NOT_POW2;
+    if (N < 3 || d->size != 1) return HZO_ERR_ARG;
+    const int NY = 2 * Ny, n = N - 2;
+    const int Ns = hzo_elements_per_rank(d);
+    const int Ktot = Ns * (N - 1) + 1;
+    const double Lx = d->xr - d->xl, Ly = d->yr - d->yl, Lz = d->zr - d->zl;
+    const size_t el = (size_t)2 * Nx * NY * N, sel = (size_t)Nx * NY * N, pl = (size_t)Nx * NY;
+    int status = HZO_OK;
+    rank_state st;
+    memset(&st, 0, sizeof(st));
+    rank_state *s = &st;
+    double *rhs1 = NULL, *etal = NULL, *etar = NULL;
+    s->ul = calloc(Ns * sel, 8); s->ur = calloc(Ns * sel, 8);
+    s->zetal = calloc(Ns * pl, 8); s->zetar = calloc(Ns * pl, 8);
+    s->b = calloc(Ns * pl, 8); s->c = calloc(Ns * pl, 8); s->rhs = calloc(Ns * pl, 8);
+    s->rhsPDE = calloc(Ns * el, 8); s->U = calloc(Ns * el, 8);
+    s->Dz = calloc((size_t)N * N, 8); s->Dzz = calloc((size_t)N * N, 8);
+    s->Op = calloc((size_t)n * n, 8); s->npivot = calloc((size_t)n, sizeof(int));
+    rhs1 = calloc(Ns * pl, 8); etal = calloc(Ns * pl, 8); etar = calloc(Ns * pl, 8);
+    if (!s->ul || !s->ur || !s->zetal || !s->zetar || !s->b || !s->c || !s->rhs || !s->rhsPDE || !s->U ||
+        !s->Dz || !s->Dzz || !s->Op || !s->npivot || !rhs1 || !etal || !etar) { status = HZO_ERR_ALLOC; goto done; }
+    status = hzo_plan_operators(N, Ns, 1, 0, Lz, NULL, s->Dz, s->Dzz);
+    if (status) goto done;
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k)
+                    s->rhsPDE[kk * el + ((size_t)i * NY + j) * N + k] =
+                        f[((size_t)i * Ny + j) * Ktot + (size_t)kk * (N - 1) + k];
+    /* Step 1 (:236-238) and Step 2 (:240-318) */
+    for (int kk = 0; kk < Ns; ++kk) hzo_fft_xy(s->rhsPDE + kk * el, 1, Nx, Ny, N);
+    for (int iw = 0; iw <= Nx / 2 && !status; iw++)
+        for (int jw = 0; jw <= Ny / 2 && !status; jw++) {
+            double shift = hzo_shift(d->lambda, iw, jw, Lx, Ly);
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b)
+                    s->Op[(size_t)a * n + b] = s->Dzz[(size_t)(a + 1) * N + (b + 1)] - (a == b ? shift : 0.0);
+            double det;
+            status = hzo_lu_decompose(s->Op, s->npivot, &det, n);
+            if (status) break;
+            int is[2], js[4], ni, nj;
+            if (iw == 0 || iw == Nx / 2) { is[0] = iw; ni = 1; }
+            else { is[0] = iw; is[1] = Nx - iw; ni = 2; }
+            if (jw == 0) { js[0] = 0; js[1] = 1; nj = 2; }
+            else if (jw == Ny / 2) { js[0] = Ny; js[1] = Ny + 1; nj = 2; }
+            else { js[0] = 2 * jw; js[1] = 2 * jw + 1; js[2] = 2 * (Ny - jw); js[3] = 2 * (Ny - jw) + 1; nj = 4; }
+            for (int a = 0; a < ni; ++a)
+                for (int b = 0; b < nj; ++b) fundamental_solutions(s, is[a], js[b], Nx, NY, N, Ns);
+        }
+    if (status) goto done;
+#define PL(arr, kk, i, j) arr[(size_t)(kk) * pl + (size_t)(i) * NY + (j)]
+#define COLU(kk, i, j) (s->U + (size_t)(kk) * el + ((size_t)(i) * NY + (j)) * N)
+#define COLL(arr, kk, i, j) (arr + (size_t)(kk) * sel + ((size_t)(i) * NY + (j)) * N)
+#define FD(k, h) first_derivative((k), (h), s->Dz, N)
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < NY; ++j) {
+            if (Ns == 1) {
+                /* :323-333 and :655-664 */
+                double b = (FD(0, COLL(s->ul, 0, i, j)) + FD(0, COLL(s->ur, 0, i, j))) -
+                           (FD(N - 1, COLL(s->ul, 0, i, j)) + FD(N - 1, COLL(s->ur, 0, i, j)));
+                double r0 = FD(N - 1, COLU(0, i, j)) - FD(0, COLU(0, i, j));
+                PL(s->zetal, 0, i, j) = r0 / b;
+                PL(s->zetar, 0, i, j) = PL(s->zetal, 0, i, j);
+                continue;
+            }
+            /* forward sweep :336-376 */
+            const int e = Ns - 1;
+            PL(s->b, 0, i, j) = FD(N - 1, COLL(s->ur, e, i, j)) - FD(N - 1, COLL(s->ul, e, i, j)) -
+                                FD(0, COLL(s->ul, 0, i, j));
+            PL(s->c, 0, i, j) = -FD(0, COLL(s->ur, 0, i, j));
+            PL(s->rhs, 0, i, j) = FD(0, COLU(0, i, j)) - FD(N - 1, COLU(e, i, j));
+            PL(rhs1, 0, i, j) = FD(N - 1, COLL(s->ul, e, i, j));
+            for (int kk = 1; kk < Ns - 1; ++kk) {
+                const double lb = FD(N - 1, COLL(s->ul, kk - 1, i, j));
+                PL(s->b, kk, i, j) = FD(N - 1, COLL(s->ur, kk - 1, i, j)) - FD(0, COLL(s->ul, kk, i, j)) -
+                                     (PL(s->c, kk - 1, i, j) / PL(s->b, kk - 1, i, j)) * lb;
+                PL(s->c, kk, i, j) = -FD(0, COLL(s->ur, kk, i, j));
+                PL(s->rhs, kk, i, j) = FD(0, COLU(kk, i, j)) - FD(N - 1, COLU(kk - 1, i, j)) -
+                                       (PL(s->rhs, kk - 1, i, j) / PL(s->b, kk - 1, i, j)) * lb;
+                PL(rhs1, kk, i, j) = -(PL(rhs1, kk - 1, i, j) / PL(s->b, kk - 1, i, j)) * lb;
+            }
+            {
+                const double lb = FD(N - 1, COLL(s->ul, e - 1, i, j));
+                PL(s->b, e, i, j) = FD(N - 1, COLL(s->ur, e - 1, i, j)) - FD(0, COLL(s->ul, e, i, j)) +
+                                    FD(0, COLL(s->ur, e, i, j)) - (PL(s->c, e - 1, i, j) / PL(s->b, e - 1, i, j)) * lb;
+                PL(s->rhs, e, i, j) = FD(0, COLU(e, i, j)) - FD(N - 1, COLU(e - 1, i, j)) -
+                                      (PL(s->rhs, e - 1, i, j) / PL(s->b, e - 1, i, j)) * lb;
+                PL(rhs1, e, i, j) = -FD(0, COLL(s->ur, e, i, j)) - (PL(rhs1, e - 1, i, j) / PL(s->b, e - 1, i, j)) * lb;
+            }
+            /* backward sweep :666-694 */
+            PL(s->zetal, e, i, j) = PL(s->rhs, e, i, j) / PL(s->b, e, i, j);
+            PL(etal, e, i, j) = PL(rhs1, e, i, j) / PL(s->b, e, i, j);
+            for (int kk = Ns - 2; kk >= 0; --kk) {
+                PL(s->zetar, kk, i, j) = PL(s->zetal, kk + 1, i, j);
+                PL(etar, kk, i, j) = PL(etal, kk + 1, i, j);
+                PL(s->zetal, kk, i, j) = (PL(s->rhs, kk, i, j) - PL(s->c, kk, i, j) * PL(s->zetar, kk, i, j)) /
+                                         PL(s->b, kk, i, j);
+                PL(etal, kk, i, j) = (PL(rhs1, kk, i, j) - PL(s->c, kk, i, j) * PL(etar, kk, i, j)) / PL(s->b, kk, i, j);
+            }
+            PL(s->zetar, e, i, j) = PL(s->zetal, 0, i, j);
+            PL(etar, e, i, j) = PL(etal, 0, i, j);
+            /* closure :698-711 */
+            const double alpha = -(PL(s->zetal, e, i, j) + PL(s->zetar, e, i, j)) /
+                                 (1.0 + PL(etal, e, i, j) + PL(etar, e, i, j));
+            for (int kk = 0; kk < Ns; ++kk) {
+                PL(s->zetal, kk, i, j) += alpha * PL(etal, kk, i, j);
+                PL(s->zetar, kk, i, j) += alpha * PL(etar, kk, i, j);
+            }
+        }
+    /* Step 5 :854-859, Step 6 :861-862 */
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < NY; ++j) {
+                double *uc = COLU(kk, i, j);
+                const double *ulc = COLL(s->ul, kk, i, j), *urc = COLL(s->ur, kk, i, j);
+                const double zl = PL(s->zetal, kk, i, j), zr = PL(s->zetar, kk, i, j);
+                for (int k = 0; k < N; ++k) uc[k] += zl * ulc[k] + zr * urc[k];
+            }
+    for (int kk = 0; kk < Ns; ++kk) hzo_fft_xy(s->U + kk * el, -1, Nx, Ny, N);
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k)
+                    u[((size_t)i * Ny + j) * Ktot + (size_t)kk * (N - 1) + k] = s->U[kk * el + ((size_t)i * NY + j) * N + k];
+#undef PL
+#undef COLU
+#undef COLL
+#undef FD
+done:
+    free_rank(s);
+    free(rhs1);
+    free(etal);
+    free(etar);
+    return status;
+}
+
 /* NonPeriodic3D_MPI.cpp:632-683 on the global layout. */
 int hzo_derivative_z(const hzo_desc *d, int order, const double *u, double *out)
 {

@@ -90,6 +90,8 @@ int hzo_solve(const hzo_desc *d, const double *f, const double *gl,
 /* FirstDerivativeZ / SecondDerivativeZ, NonPeriodic3D_MPI.cpp:632-683, on
  * the global layout (element-wise application of the scaled matrices). */
 int hzo_derivative_z(const hzo_desc *d, int order, const double *u, double *out);
+/* Periodic3D_MPI.cpp SolvePDE at size = 1: u[Nx][Ny][Ktot] from f (z periodic). */
+int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,7 @@ def lib():
         L.hzo_z_grid.argtypes = [ctypes.POINTER(Desc), _D]
         L.hzo_solve.argtypes = [ctypes.POINTER(Desc), _D, _D, _D, _D]
         L.hzo_derivative_z.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, _D, _D]
+        L.hzo_solve_periodic.argtypes = [ctypes.POINTER(Desc), _D, _D]
         _lib = L
     return _lib
 
@@ -152,6 +153,15 @@ def solve(desc, f, gl, gr):
     gr = np.ascontiguousarray(gr, dtype=np.float64).reshape(desc.Nx, desc.Ny)
     u = np.zeros_like(f)
     _check(lib().hzo_solve(ctypes.byref(desc), _p(f), _p(gl), _p(gr), _p(u)))
+    return u
+
+
+def solve_periodic(desc, f):
+    """Reference periodic-z SolvePDE (Periodic3D_MPI.cpp) at size = 1."""
+    K = total_z_points(desc)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(desc.Nx, desc.Ny, K)
+    u = np.zeros_like(f)
+    _check(lib().hzo_solve_periodic(ctypes.byref(desc), _p(f), _p(u)))
     return u
 
 

@@ -10,6 +10,7 @@ plus sampled values and norms (large cases).
 
     python tests/golden/make_golden.py              # all but the largest case
     python tests/golden/make_golden.py --config4     # 512 x 512 x 257 (minutes)
+    python tests/golden/make_golden.py --periodic    # the periodic-z solver
 """
 import hashlib
 import json
@@ -30,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 from tests_lu_cases import lu_cases  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REFP = os.path.join(ROOT, "oracle", "_ref", "ref_driver_periodic")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 TMP = tempfile.mkdtemp(prefix="golden_")
 
@@ -117,6 +119,28 @@ def ref_solve(desc, f, gl, gr):
     return u, secs
 
 
+def ref_solve_periodic(desc, f):
+    """Periodic3D_MPI SolvePDE under mpiexec -np desc.size."""
+    K = po.total_z_points(desc)
+    inp = os.path.join(TMP, "p_in.bin")
+    outp = os.path.join(TMP, "p_out.bin")
+    np.ascontiguousarray(f).ravel().tofile(inp)
+    args = [MPIEXEC, "-np", str(desc.size), REFP, "psolve", str(desc.Nx), str(desc.Ny), str(desc.Nz), str(desc.N)]
+    args += [repr(v) for v in (desc.xl, desc.xr, desc.yl, desc.yr, desc.zl, desc.zr, desc.lam)]
+    args += [inp, outp]
+    res = run(args)
+    Ns = (K - 1) // (desc.size * (desc.N - 1))
+    N = desc.N
+    u = np.zeros((desc.Nx, desc.Ny, K))
+    for r in range(desc.size):
+        a = np.fromfile(f"{outp}.rank{r}")
+        blk = a[: Ns * desc.Nx * desc.Ny * N].reshape(Ns, desc.Nx, desc.Ny, N)
+        for kk in range(Ns):
+            g = (r * Ns + kk) * (N - 1)
+            u[:, :, g:g + N] = blk[kk]
+    return u, float(res.stdout.split()[1])
+
+
 def sample_idx(shape, count, seed):
     rng = wl.SplitMix64(seed)
     return [[rng.randint(0, s - 1) for s in shape] for _ in range(count)]
@@ -136,6 +160,40 @@ def solve_case(name, desc, f, gl, gr, exact=None, full=False, gen=None):
     with open(np_path, "w") as fp:
         json.dump(rec, fp, indent=0)
     print(f"{name}: ref {secs:.3f}s err_vs_exact {rec.get('err_vs_exact')}")
+
+
+def periodic_case(name, desc, f, exact=None, gen=None):
+    u, secs = ref_solve_periodic(desc, f)
+    rec = {"desc": {k: getattr(desc, k) for k, _ in desc._fields_}, "generator": gen, "periodic": True,
+           "sha_f": sha(f), "sha_u": sha(u), "max_abs_u": float(np.abs(u).max()), "ref_seconds": secs}
+    if exact is not None:
+        rec["err_vs_exact"] = float(np.abs(u - exact).max())
+    idx = sample_idx(u.shape, 4096, 99)
+    rec["sample_idx"] = idx
+    rec["sample_u"] = hexs([u[i, j, k] for i, j, k in idx])
+    with open(os.path.join(HERE, f"psolve_{name}.json"), "w") as fp:
+        json.dump(rec, fp, indent=0)
+    print(f"periodic {name}: ref {secs:.3f}s err_vs_exact {rec.get('err_vs_exact')}")
+
+
+def periodic_shipped_inputs(size):
+    """Inputs of the reference's periodic test program (TestPeriodic3D_MPI.cpp:27-41, :96-107)."""
+    PI = 3.1415926535897932
+    d = po.make_desc(64, 32, 250, 10, size=size, xl=0, xr=1, yl=0, yr=1, zl=0, zr=1, lam=1.0)
+    z = po.z_grid(d)
+    x = np.arange(64) / 64.0
+    y = np.arange(32) / 32.0
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    f = -(64 * PI * PI + 36 * PI * PI + 16 * PI * PI + 1.0) * np.sin(8 * PI * X) * np.cos(6 * PI * Y) * np.cos(4 * PI * Z)
+    ex = np.sin(8 * PI * X) * np.cos(6 * PI * Y) * np.cos(4 * PI * Z)
+    return d, np.ascontiguousarray(f), ex
+
+
+def periodic_random_inputs(nx, N, nz, lam, seed):
+    d = po.make_desc(nx, nx, nz, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1, lam=lam)
+    x, y, zeta = wl.grid(nx, nx, po.z_grid(d))
+    f, _, _ = wl.random_problem(x, y, zeta, seed=seed, nmodes=32, kmax=nx // 4, lmax=min(N - 1, 32))
+    return d, np.ascontiguousarray(f)
 
 
 def shipped_inputs(size):
@@ -183,6 +241,15 @@ def config4_inputs():
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if "--periodic" in sys.argv:
+        for size in (1, 4):
+            d, f, ex = periodic_shipped_inputs(size)
+            periodic_case(f"shipped_np{size}", d, f, ex, gen="TestPeriodic3D_MPI.cpp inputs")
+        for nx, N, nz, lam, seed in ((64, 33, 129, 10.0, 7), (128, 129, 129, 10.0, 8), (32, 9, 33, 0.5, 9)):
+            d, f = periodic_random_inputs(nx, N, nz, lam, seed)
+            periodic_case(f"random_{nx}_N{N}", d, f,
+                          gen=f"random_problem(seed={seed}, nmodes=32, kmax={nx // 4}, lmax={min(N - 1, 32)}) lam={lam}")
+        return
     if "--config4" in sys.argv:  # minutes of CPU time: on request only
         d, f, gl, gr = config4_inputs()
         solve_case("config4_N257_np1", d, f, gl, gr,
