@@ -80,6 +80,10 @@ typedef struct {
     double mixed_fp64_cond;  /* MIXED only: shift sigma below which a mode keeps
                                 an FP64 operator (0 = built-in rule:
                                 cond * 2^-24 > 1)                        */
+    int periodic_z;          /* 1: z periodic as in Periodic3D_MPI.cpp (u and
+                                u_z continuous across z = zr -> zl); the Robin
+                                coefficients and face data are not used.
+                                PLAIN_LU and DIAG forms.                   */
 } hz_plan_desc;
 
 /* Replaces CreatePDEPlan (NonPeriodic3D_MPI.cpp:40-134): GLL nodes, scaled
@@ -102,9 +106,11 @@ int hz_plan_set_lambda(hz_plan *plan, double lambda);
 /* Changes the Robin coefficients of the two faces for later solves. */
 int hz_plan_set_robin(hz_plan *plan, double alpha_l, double beta_l, double alpha_r, double beta_r);
 
-/* Replaces SolvePDE (NonPeriodic3D_MPI.cpp:237-591) for the standard
- * operator of the reference test callback (TestNonPeriodic3D_MPI.cpp:71-79):
+/* Replaces SolvePDE (NonPeriodic3D_MPI.cpp:237-591; with periodic_z,
+ * Periodic3D_MPI.cpp:233-863) for the standard operator of the reference
+ * test callback (TestNonPeriodic3D_MPI.cpp:71-79):
  * Dzz_int - (lambda + 4 pi^2 iw^2/Lx^2 + 4 pi^2 jw^2/Ly^2) I.
+ * g_l, g_r may be NULL for a periodic-z plan (they are not used).
  * Inputs are not modified (the reference destroys them). */
 int hz_solve(hz_plan *plan, const double *f, const double *g_l, const double *g_r,
              double *u, hz_stream_t stream);

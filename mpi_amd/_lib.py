@@ -35,7 +35,8 @@ class PlanDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("n", ctypes.c_int), ("ne", ctypes.c_int),
                 ("xl", _D), ("xr", _D), ("yl", _D), ("yr", _D), ("zl", _D), ("zr", _D),
                 ("lam", _D), ("alpha_l", _D), ("beta_l", _D), ("alpha_r", _D), ("beta_r", _D),
-                ("form", ctypes.c_int), ("refine_sweeps", ctypes.c_int), ("mixed_fp64_cond", _D)]
+                ("form", ctypes.c_int), ("refine_sweeps", ctypes.c_int), ("mixed_fp64_cond", _D),
+                ("periodic_z", ctypes.c_int)]
 
 
 SIGNATURES = {

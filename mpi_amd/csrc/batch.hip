@@ -448,6 +448,7 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         for (int q = 0; q < p; ++q)
             if (plans[q] == pl) return set_error(HZ_ERR_ARG, "a plan appears twice in the batch");
         if (pl->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG-form plans only");
+        if (pl->d.periodic_z) return set_error(HZ_ERR_UNSUPPORTED, "batch: periodic z not supported");
         if (pl->device != dev) return set_error(HZ_ERR_ARG, "batch: plans on another device");
         if (pl->n > 128) return set_error(HZ_ERR_UNSUPPORTED, "batch: N-2 must be <= 128");
         if (pl->nx < 16 || pl->nx > 256 || pl->ny < 16 || pl->ny > 256)

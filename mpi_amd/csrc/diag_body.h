@@ -36,6 +36,7 @@
 #pragma once
 
 #include "hz_internal.h"
+#include "sweep.h"
 
 namespace hz {
 namespace {
@@ -277,11 +278,16 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             const int i = fc / M, m = fc % M;
             const int iw = i <= a.nx / 2 ? i : a.nx - i;
             const int mode = iw * M + m;
-            const double *bsrc = plane ? a.bc_im : a.bc_re;
-            const double gl = bsrc[fc], gr = bsrc[(long long)nfc + fc];
             const double d0ul = a.hom[4 * mode + 0], dNul = a.hom[4 * mode + 1];
             const double d0ur = a.hom[4 * mode + 2], dNur = a.hom[4 * mode + 3];
             const int r0w = (fl * 2 + plane) * E;
+            if (a.periodic) {
+                periodic_sweep(E, d0ul, dNul, d0ur, dNur, row_d0 + r0w, row_dN + r0w, row_zl + r0w,
+                               row_zr + r0w, 1);
+                continue;
+            }
+            const double *bsrc = plane ? a.bc_im : a.bc_re;
+            const double gl = bsrc[fc], gr = bsrc[(long long)nfc + fc];
             double bprev = -a.alpha_l * d0ul + a.beta_l;
             double cprev = -a.alpha_l * d0ur;
             double rprev = gl + a.alpha_l * row_d0[r0w];

@@ -78,6 +78,7 @@ struct LuModeArgs {
     const double *bc_re, *bc_im;
     double alpha_l, beta_l, alpha_r, beta_r;
     int *status;             // [modes]
+    int periodic;            // z periodic: cyclic interface system, no face data
 };
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s);
 
@@ -95,6 +96,7 @@ struct DiagArgs {
     int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
     double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
     int rt;                     // tile rows (spectral columns x planes x elements); 0 = default
+    int periodic;               // z periodic: cyclic interface system, no face data
 };
 int diag_padded(int n);
 // Tile heights offered for padded size NP (autotuned per plan).

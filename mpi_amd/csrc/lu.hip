@@ -34,6 +34,7 @@
 // a register array.
 #include "hz_internal.h"
 #include "lu_slots.h"
+#include "sweep.h"
 
 namespace hz {
 namespace {
@@ -450,7 +451,11 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     __syncthreads();
 
     // ---- element-interface sweep, one thread per real column ----
-    if (threadIdx.x < nc) {
+    if (threadIdx.x < nc && a.periodic) {
+        const int cc = threadIdx.x;
+        periodic_sweep(E, dots[0][0], dots[1][0], dots[0][1], dots[1][1], &dots[0][2 + cc], &dots[1][2 + cc],
+                       &zeta[0][cc], &zeta[1][cc], 4);
+    } else if (threadIdx.x < nc) {
         const int cc = threadIdx.x;
         const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
         const double *bsrc = (cc & 1) ? a.bc_im : a.bc_re;

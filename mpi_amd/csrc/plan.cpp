@@ -420,6 +420,7 @@ DiagArgs hz::diag_args(const hz_plan *p)
     a.alpha_r = p->d.alpha_r;
     a.beta_r = p->d.beta_r;
     a.rt = p->diag_rt;
+    a.periodic = p->d.periodic_z;
     return a;
 }
 
@@ -571,6 +572,8 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     if (d.ne < 1) return set_error(HZ_ERR_ARG, "ne must be >= 1");
     if (!(d.xr > d.xl) || !(d.yr > d.yl) || !(d.zr > d.zl)) return set_error(HZ_ERR_ARG, "empty domain");
     if (d.form < HZ_FORM_PLAIN_LU || d.form > HZ_FORM_MIXED) return set_error(HZ_ERR_ARG, "unknown form");
+    if (d.periodic_z && d.form == HZ_FORM_MIXED)
+        return set_error(HZ_ERR_UNSUPPORTED, "periodic z: PLAIN_LU and DIAG forms only");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(HZ_ERR_NO_DEVICE, "no HIP device");
 
@@ -673,7 +676,12 @@ int hz_plan_set_robin(hz_plan *p, double alpha_l, double beta_l, double alpha_r,
 static int solve_impl(hz_plan *p, const double *f, const double *gl, const double *gr,
                       const double *op_diag, const double *op_full, double *u, hipStream_t s)
 {
-    if (!p || !f || !gl || !gr || !u) return set_error(HZ_ERR_ARG, "null argument");
+    if (!p || !f || !u) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->d.periodic_z) {
+        gl = gr = nullptr;  // no face data in the periodic problem
+    } else if (!gl || !gr) {
+        return set_error(HZ_ERR_ARG, "null face data");
+    }
     StageRecorder rec(p, s);
     int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr);
     rec.mark("fft_forward");
@@ -702,6 +710,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.alpha_r = p->d.alpha_r;
         a.beta_r = p->d.beta_r;
         a.status = p->d_status;
+        a.periodic = p->d.periodic_z;
         st = launch_lu_modes(a, p->modes, s);
         rec.mark("lu_modes");
     } else if (p->d.form == HZ_FORM_DIAG) {
@@ -820,6 +829,7 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
 {
     if (!p || !ubar || !fbar || !glbar || !grbar) return set_error(HZ_ERR_ARG, "null argument");
     if (p->ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: one element only");
+    if (p->d.periodic_z) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: periodic z not supported");
     if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
     const hipStream_t s = (hipStream_t)stream;
     int st;

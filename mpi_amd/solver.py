@@ -33,13 +33,13 @@ def _stream_ptr(stream):
 class HelmholtzPlan:
     def __init__(self, nx, ny, n, ne=1, xl=0.0, xr=2 * math.pi, yl=0.0, yr=2 * math.pi,
                  zl=-1.0, zr=1.0, lam=1.0, alpha_l=0.0, beta_l=1.0, alpha_r=0.0, beta_r=1.0,
-                 form="plain", refine_sweeps=0, mixed_fp64_cond=0.0):
+                 form="plain", refine_sweeps=0, mixed_fp64_cond=0.0, periodic_z=False):
         if not torch.cuda.is_available():
             raise RuntimeError("HelmholtzPlan needs a HIP device")
         self.lib = load()
         self.desc = _lib.PlanDesc(nx, ny, n, ne, xl, xr, yl, yr, zl, zr, lam, alpha_l, beta_l,
                                   alpha_r, beta_r, FORMS[form] if isinstance(form, str) else form,
-                                  refine_sweeps, mixed_fp64_cond)
+                                  refine_sweeps, mixed_fp64_cond, 1 if periodic_z else 0)
         self.handle = ctypes.c_void_p()
         check(self.lib.hz_plan_create(ctypes.byref(self.desc), ctypes.byref(self.handle)))
         k = ctypes.c_int()
@@ -83,10 +83,12 @@ class HelmholtzPlan:
         if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
             raise ValueError(f"expected contiguous float64 HIP tensor of shape {shape}")
 
-    def solve(self, f, gl, gr, out=None, stream=None):
+    def solve(self, f, gl=None, gr=None, out=None, stream=None):
+        """u for right-hand side f and face data gl, gr (None for a periodic-z plan)."""
         self._check_field(f, (self.nx, self.ny, self.nz))
-        self._check_field(gl, (self.nx, self.ny))
-        self._check_field(gr, (self.nx, self.ny))
+        if not self.desc.periodic_z:
+            self._check_field(gl, (self.nx, self.ny))
+            self._check_field(gr, (self.nx, self.ny))
         u = out if out is not None else self.empty_field()
         check(self.lib.hz_solve(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(u), _stream_ptr(stream)))
         return u

@@ -1,0 +1,76 @@
+"""Periodic-z solver (Periodic3D_MPI.cpp) on the GPU, PLAIN_LU and DIAG forms,
+against the reference's own output (golden vectors) and the oracle.
+
+Bars: PLAIN <= 1e-12, DIAG <= 1e-10 relative max-norm error.
+"""
+import numpy as np
+import pytest
+
+from golden_util import load, periodic_case_inputs, sample
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+TOL = {"plain": 1e-12, "diag": 1e-10}
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def plan_for(d, ne, form):
+    from mpi_amd.solver import HelmholtzPlan
+    return HelmholtzPlan(d["Nx"], d["Ny"], d["N"], ne, xl=d["xl"], xr=d["xr"], yl=d["yl"], yr=d["yr"],
+                         zl=d["zl"], zr=d["zr"], lam=d["lam"], form=form, periodic_z=True)
+
+
+@pytest.mark.parametrize("form", ["plain", "diag"])
+@pytest.mark.parametrize("name", ["shipped_np1", "random_64_N33", "random_128_N129", "random_32_N9"])
+def test_periodic_matches_reference(cuda, name, form):
+    rec = load(f"psolve_{name}.json")
+    d = rec["desc"]
+    desc = po.make_desc(d["Nx"], d["Ny"], d["Nz"], d["N"], size=1, xl=d["xl"], xr=d["xr"], yl=d["yl"],
+                        yr=d["yr"], zl=d["zl"], zr=d["zr"], lam=d["lam"])
+    K = po.total_z_points(desc)
+    ne = (K - 1) // (d["N"] - 1)
+    plan = plan_for(d, ne, form)
+    assert plan.nz == K
+    rec, f, ex = periodic_case_inputs(name, po.z_grid(desc))
+    u = plan.solve(dev(f)).cpu().numpy()
+    got, want = sample(u, rec)
+    err = float(np.abs(got - want).max() / rec["max_abs_u"])
+    print(f"{name} {form}: rel err vs reference {err:.3e}")
+    assert err <= TOL[form]
+    if ex is not None:
+        assert np.abs(u - ex).max() <= 10 * rec["err_vs_exact"] + 1e-13
+    if K < 200:
+        uo = po.solve_periodic(desc, f)
+        assert float(np.abs(u - uo).max() / np.abs(uo).max()) <= TOL[form]
+    plan.close()
+
+
+@pytest.mark.parametrize("form", ["plain", "diag"])
+def test_periodic_four_rank_reference(cuda, form):
+    """The reference at np=4 (4 x 7 elements of N = 10) against a 28-element plan."""
+    rec = load("psolve_shipped_np4.json")
+    d = rec["desc"]
+    desc4 = po.make_desc(d["Nx"], d["Ny"], d["Nz"], d["N"], size=4, xl=d["xl"], xr=d["xr"], yl=d["yl"],
+                         yr=d["yr"], zl=d["zl"], zr=d["zr"], lam=d["lam"])
+    plan = plan_for(d, 28, form)
+    rec, f, ex = periodic_case_inputs("shipped_np4", po.z_grid(desc4))
+    u = plan.solve(dev(f)).cpu().numpy()
+    got, want = sample(u, rec)
+    assert float(np.abs(got - want).max() / rec["max_abs_u"]) <= TOL[form]
+    plan.close()
+
+
+def test_periodic_rejects_unsupported(cuda):
+    import torch
+    from mpi_amd._lib import HzError
+    from mpi_amd.solver import HelmholtzPlan
+    with pytest.raises(HzError):
+        HelmholtzPlan(8, 8, 9, 1, form="mixed", periodic_z=True)
+    p = HelmholtzPlan(8, 8, 9, 1, form="diag", periodic_z=True)
+    with pytest.raises(HzError):
+        p.adjoint(torch.zeros((8, 8, p.nz), dtype=torch.float64, device="cuda"))
+    p.close()
