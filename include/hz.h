@@ -148,6 +148,16 @@ int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
 int hz_derivative_z(hz_plan *plan, int order, const double *u, double *out,
                     hz_stream_t stream);
 
+/* Fourier derivatives along x (axis 0) or y (axis 1), order 1 or 2, of a
+ * stack of `lanes` planes u[nx][ny][lanes] -> out (device pointers): the
+ * 3-D FirstDerivativeFourierX/Y and SecondDerivativeFourierX/Y of
+ * Fourier_Quadrature.cpp:1494-1575, :1721-1800 (wavenumber k = j for
+ * j < L/2, j - L otherwise), which the reference's FirstDerivativeX/Y and
+ * SecondDerivativeX/Y of the Helmholtz solver leave empty
+ * (NonPeriodic3D_MPI.cpp:596-627). */
+int hz_derivative_xy(hz_plan *plan, int axis, int order, const double *u, double *out, int lanes,
+                     hz_stream_t stream);
+
 /* Replaces FourierTransformXY (Fourier_Quadrature.cpp:1040-1155) for a
  * stack of `lanes` planes: isign = +1 maps real in[nx][ny][lanes] to the half
  * spectrum re/im[nx][ny/2+1][lanes] (unnormalised, exp(-2 pi i jk/L), x index

@@ -38,6 +38,13 @@ void DestroyPDEPlan();
 
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int, const int));
 
+// The reference leaves the X/Y versions empty (NonPeriodic3D_MPI.cpp:596-627);
+// here they are the Fourier derivatives of Fourier_Quadrature.cpp:1494-1800
+// applied to every element, on the GPU.
+void FirstDerivativeX(FieldVariable Ux[], FieldVariable U[]);
+void SecondDerivativeX(FieldVariable Uxx[], FieldVariable U[]);
+void FirstDerivativeY(FieldVariable Uy[], FieldVariable U[]);
+void SecondDerivativeY(FieldVariable Uyy[], FieldVariable U[]);
 void FirstDerivativeZ(FieldVariable Uz[], FieldVariable U[]);
 void SecondDerivativeZ(FieldVariable Uzz[], FieldVariable U[]);
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "hz_solve_status": (ctypes.c_int, [_VP, _VP]),
     "hz_adjoint": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _PD, _VP]),
     "hz_derivative_z": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP]),
+    "hz_derivative_xy": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_int, _VP]),
     "hz_fourier_xy": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, ctypes.c_int, _VP]),
     "hz_lu_factor_batched": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP, _VP]),
     "hz_lu_solve_batched": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_int, _VP]),

@@ -85,7 +85,36 @@ void derivative(FieldVariable out[], FieldVariable in[], int order)
                 std::memcpy(out[kk].phi[i][j], &host[(((size_t)kk * Nx + i) * Ny + j) * col], sizeof(double) * N);
 }
 
+// x/y Fourier derivative of every element: gather to [Nx][Ny][Ns*N] (z lanes
+// innermost), one hz_derivative_xy call, scatter back.
+void derivative_xy(FieldVariable out[], FieldVariable in[], int axis, int order)
+{
+    const size_t lanes = (size_t)Ns * N;
+    std::vector<double> host((size_t)Nx * Ny * lanes);
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j)
+            for (int kk = 0; kk < Ns; ++kk)
+                std::memcpy(&host[((size_t)i * Ny + j) * lanes + (size_t)kk * N], in[kk].phi[i][j], sizeof(double) * N);
+    double *d_in = nullptr, *d_out = nullptr;
+    hip_check(hipMalloc(&d_in, sizeof(double) * host.size()), "hipMalloc");
+    hip_check(hipMalloc(&d_out, sizeof(double) * host.size()), "hipMalloc");
+    hip_check(hipMemcpy(d_in, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hz_check(hz_derivative_xy(g_plan, axis, order, d_in, d_out, (int)lanes, nullptr));
+    hip_check(hipMemcpy(host.data(), d_out, sizeof(double) * host.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j)
+            for (int kk = 0; kk < Ns; ++kk)
+                std::memcpy(out[kk].phi[i][j], &host[((size_t)i * Ny + j) * lanes + (size_t)kk * N], sizeof(double) * N);
+}
+
 }  // namespace
+
+void FirstDerivativeX(FieldVariable Ux[], FieldVariable U[]) { derivative_xy(Ux, U, 0, 1); }
+void SecondDerivativeX(FieldVariable Uxx[], FieldVariable U[]) { derivative_xy(Uxx, U, 0, 2); }
+void FirstDerivativeY(FieldVariable Uy[], FieldVariable U[]) { derivative_xy(Uy, U, 1, 1); }
+void SecondDerivativeY(FieldVariable Uyy[], FieldVariable U[]) { derivative_xy(Uyy, U, 1, 2); }
 
 // General.cpp:6-11
 void ErrorMessage(const char *message)

@@ -105,6 +105,16 @@ int main(int argc, char **argv)
                     derr = Maximum(derr, Absolute(W[kk].phi[i][j][k] +
                                                   16.0 * PI * PI * ExactFunction(xl + i * dx, yl + j * dy, z[kk * (N - 1) + k])));
     std::cout << "SecondDerivativeZ error = " << derr << std::endl;
+    // and along x: the exact solution's x dependence is sin(8 pi x)
+    SecondDerivativeX(W, V);
+    derr = 0.0;
+    for (int kk = 0; kk < Ns; ++kk)
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k)
+                    derr = Maximum(derr, Absolute(W[kk].phi[i][j][k] +
+                                                  64.0 * PI * PI * ExactFunction(xl + i * dx, yl + j * dy, z[kk * (N - 1) + k])));
+    std::cout << "SecondDerivativeX error = " << derr << std::endl;
 
     for (int kk = 0; kk < Ns; ++kk) {
         Deallocate(V[kk].phi, 2 * Nx, NY, N);

@@ -140,6 +140,10 @@ struct LuFactorArgs {
 int launch_lu_factor(const LuFactorArgs &a, hipStream_t s);
 int launch_lu_solve(const double *LU, const int *piv, double *b, int n, int count, hipStream_t s);
 int launch_derivative(const double *D, int N, long long ncols, const double *u, double *out, hipStream_t s);
+// Multiplies the half spectrum re/im[A][B][lanes] by (2 pi i k/len)^order along
+// axis A (axis_a) or B of length L (Fourier derivative).
+int launch_spectral_derivative(double *re, double *im, long long A, long long B, long long lanes, int axis_a, int L,
+                               double len, int order, hipStream_t s);
 // sum_i a[i]*b[i] into *out (device), deterministic two-pass reduction; partial >= 1024 doubles
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
 
@@ -181,6 +185,8 @@ struct hz_plan {
     double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass
     std::vector<double> diag_rt_ms;                   // tuning times per choice
     double *d_partial = nullptr;                      // reduction scratch
+    double *d_xspec = nullptr;                        // half spectrum of the x/y derivatives
+    size_t xspec_n = 0;
     // MIXED form
     int ldx = 0, n32 = 0, n64 = 0, sweeps = 0;
     double condV = 0.0;

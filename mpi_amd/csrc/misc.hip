@@ -55,7 +55,47 @@ __global__ __launch_bounds__(256) void dot_final_kernel(const double *__restrict
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// Spectral derivative along one axis of a half spectrum re/im[A][B][lanes]
+// (the transformed axis is A when axis_a, else B, of length L): multiplies by
+// (2 pi i k / len)^order with k = j for j < L/2 and j - L otherwise, the
+// wavenumber convention of Fourier_Quadrature.cpp:1512-1519 / :1739-1745
+// (the Nyquist point counts as -L/2).
+__global__ __launch_bounds__(256) void spectral_derivative_kernel(double *__restrict__ re, double *__restrict__ im,
+                                                                  long long A, long long B, long long lanes,
+                                                                  int axis_a, int L, double len, int order)
+{
+    const long long total = A * B * lanes;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long ab = t / lanes;
+        const int j = (int)(axis_a ? ab / B : ab % B);
+        const double k = j < L / 2 ? (double)j : (double)(j - L);
+        const double c = two_pi * k / len;
+        const double r = re[t], q = im[t];
+        if (order == 1) {
+            re[t] = -c * q;
+            im[t] = c * r;
+        } else {
+            re[t] = -(c * c) * r;
+            im[t] = -(c * c) * q;
+        }
+    }
+}
+
 }  // namespace
+
+int launch_spectral_derivative(double *re, double *im, long long A, long long B, long long lanes, int axis_a, int L,
+                               double len, int order, hipStream_t s)
+{
+    const long long total = A * B * lanes;
+    const int blocks = (int)std::min<long long>(4096, (total + 255) / 256);
+    if (total > 0)
+        hipLaunchKernelGGL(spectral_derivative_kernel, dim3(blocks), dim3(256), 0, s, re, im, A, B, lanes, axis_a, L,
+                           len, order);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
 
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s)
 {

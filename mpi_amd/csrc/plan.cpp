@@ -81,6 +81,7 @@ void free_plan(hz_plan *p)
         (void)hipFree(q);
     (void)hipFree(p->d_X32);
     (void)hipFree(p->d_slot);
+    (void)hipFree(p->d_xspec);
     (void)hipFree(p->d_status);
     delete p;
 }
@@ -896,6 +897,61 @@ int hz_mixed_fp64_modes(const hz_plan *p, int *fp64_modes)
     if (!p || !fp64_modes) return set_error(HZ_ERR_ARG, "null argument");
     *fp64_modes = p->n64;
     return HZ_OK;
+}
+
+// Fourier derivative along x or y: one R2C pass along that axis into a half
+// spectrum, the wavenumber multiply, one C2R pass back (two passes over the
+// field, not the four of the 2-D transform).
+int hz_derivative_xy(hz_plan *p, int axis, int order, const double *u, double *out, int lanes,
+                     hz_stream_t stream)
+{
+    if (!p || !u || !out || lanes < 1 || (axis != 0 && axis != 1) || (order != 1 && order != 2))
+        return set_error(HZ_ERR_ARG, "bad argument");
+    const hipStream_t s = (hipStream_t)stream;
+    const long long W = lanes;
+    const long long A = axis == 0 ? p->nx / 2 + 1 : p->nx;
+    const long long B = axis == 0 ? p->ny : p->M;
+    const size_t half = (size_t)A * B * W;
+    int st;
+    if (2 * half > p->xspec_n) {
+        (void)hipFree(p->d_xspec);
+        p->d_xspec = nullptr;
+        p->xspec_n = 0;
+        if ((st = dmalloc(&p->d_xspec, 2 * half))) return st;
+        p->xspec_n = 2 * half;
+    }
+    double *re = p->d_xspec, *im = p->d_xspec + half;
+    FftArgs a{};
+    int L;
+    if (axis == 0) {  // lines along x, batch = y index; same strides on both sides
+        L = p->nx;
+        a.in_batch_stride = a.out_batch_stride = W;
+        a.in_point_stride = a.out_point_stride = (long long)p->ny * W;
+        a.nbatch = p->ny;
+        a.tw = p->d_twx;
+    } else {          // lines along y, batch = x index
+        L = p->ny;
+        a.in_batch_stride = (long long)p->ny * W;
+        a.in_point_stride = W;
+        a.out_batch_stride = (long long)p->M * W;
+        a.out_point_stride = W;
+        a.nbatch = p->nx;
+        a.tw = p->d_twy;
+    }
+    a.lanes = lanes;
+    FftArgs b = a;
+    a.in_re = u;
+    a.out_re = re;
+    a.out_im = im;
+    if ((st = launch_fft(L, FFT_R2C, a, s))) return st;
+    const double len = axis == 0 ? p->Lx : p->Ly;
+    if ((st = launch_spectral_derivative(re, im, A, B, W, axis == 0, L, len, order, s))) return st;
+    b.in_re = re;  // C2R: the half spectrum in, the physical layout out
+    b.in_im = im;
+    b.out_re = out;
+    std::swap(b.in_batch_stride, b.out_batch_stride);
+    std::swap(b.in_point_stride, b.out_point_stride);
+    return launch_fft(L, FFT_C2R, b, s);
 }
 
 int hz_host_plan_operators(int n, int ne, double lz, double *x, double *z, double *Dz, double *Dzz)

@@ -116,6 +116,15 @@ class HelmholtzPlan:
         check(self.lib.hz_derivative_z(self.handle, int(order), _ptr(u), _ptr(o), _stream_ptr(stream)))
         return o
 
+    def derivative_xy(self, u, axis, order=1, out=None, stream=None):
+        """Fourier derivative of u[nx, ny, lanes] along x (axis 0) or y (axis 1)."""
+        if u.dim() != 3 or u.shape[0] != self.nx or u.shape[1] != self.ny or not u.is_contiguous():
+            raise ValueError("expected a contiguous [nx, ny, lanes] tensor")
+        o = out if out is not None else torch.empty_like(u)
+        check(self.lib.hz_derivative_xy(self.handle, int(axis), int(order), _ptr(u), _ptr(o), u.shape[2],
+                                        _stream_ptr(stream)))
+        return o
+
     def fourier_xy(self, x, lanes=None, stream=None):
         """Forward half-spectrum transform of real x[nx, ny, lanes] -> (re, im)[nx, ny//2+1, lanes]."""
         lanes = x.shape[2] if x.dim() == 3 else 1

@@ -12,6 +12,9 @@
 //   lu n COUNT IN OUT               IN: COUNT*n*n doubles; OUT per matrix:
 //                                   LU[n*n] det, then COUNT*n int32 pivots
 //   fft Nx Ny Nz ISIGN IN OUT       FourierTransformXY on A[2Nx][2Ny][Nz]
+//   fderiv Nx Ny Nz Lx Ly IN OUT    FirstDerivativeFourierX/Y, SecondDerivative-
+//                                   FourierX/Y on F[Nx][Ny][Nz]; OUT: the four
+//                                   results in that order
 //   solve Nx Ny Nz N xl xr yl yr zl zr al bl ar br lambda IN OUT
 //                                   (run under mpiexec -np P) IN: f[Nx][Ny][Ktot]
 //                                   gl[Nx][Ny] gr[Nx][Ny]; OUT.rankR: this
@@ -182,6 +185,36 @@ static int mode_solve(int argc, char **argv)
     return 0;
 }
 
+static int mode_fderiv(int argc, char **argv)
+{
+    if (argc < 9) return 2;
+    const int nx = std::atoi(argv[2]), ny = std::atoi(argv[3]), nz = std::atoi(argv[4]);
+    const double lx = std::atof(argv[5]), ly = std::atof(argv[6]);
+    const size_t count = (size_t)nx * ny * nz;
+    std::vector<double> in = read_doubles(argv[7], count);
+    CreateFFTPlan(Maximum(nx, ny));
+    double ***F, ***G;
+    Allocate(F, nx, ny, nz);
+    Allocate(G, nx, ny, nz);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) F[i][j][k] = in[((size_t)i * ny + j) * nz + k];
+    FILE *fp = std::fopen(argv[8], "wb");
+    for (int which = 0; which < 4; ++which) {
+        if (which == 0) FirstDerivativeFourierX(G, F, nx, ny, nz, lx);
+        if (which == 1) FirstDerivativeFourierY(G, F, nx, ny, nz, ly);
+        if (which == 2) SecondDerivativeFourierX(G, F, nx, ny, nz, lx);
+        if (which == 3) SecondDerivativeFourierY(G, F, nx, ny, nz, ly);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) write_raw(fp, G[i][j], sizeof(double) * nz);
+    }
+    std::fclose(fp);
+    Deallocate(F, nx, ny, nz);
+    Deallocate(G, nx, ny, nz);
+    DestroyFFTPlan(Maximum(nx, ny));
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
@@ -190,6 +223,7 @@ int main(int argc, char **argv)
     if (mode == "lu") return mode_lu(argc, argv);
     if (mode == "fft") return mode_fft(argc, argv);
     if (mode == "solve") return mode_solve(argc, argv);
+    if (mode == "fderiv") return mode_fderiv(argc, argv);
     std::fprintf(stderr, "ref_driver: unknown mode %s\n", mode.c_str());
     return 2;
 }

@@ -11,6 +11,7 @@ plus sampled values and norms (large cases).
     python tests/golden/make_golden.py              # all but the largest case
     python tests/golden/make_golden.py --config4     # 512 x 512 x 257 (minutes)
     python tests/golden/make_golden.py --periodic    # the periodic-z solver
+    python tests/golden/make_golden.py --fderiv      # Fourier derivatives
 """
 import hashlib
 import json
@@ -93,6 +94,25 @@ def fft_fixture():
             rec[f"sha_out_{isign}"] = sha(r)
             rec[f"head_{isign}"] = hexs(r[:32])
         out[f"{nx}x{ny}x{nz}"] = rec
+    return out
+
+
+def fderiv_fixture():
+    """Fourier derivatives along x and y of 3-D arrays (Fourier_Quadrature.cpp:1494-1575, :1721-1800)."""
+    out = {}
+    rng = wl.SplitMix64(91)
+    for nx, ny, nz, lx, ly in ((16, 8, 3, 2 * math.pi, 3.0), (64, 32, 5, 1.0, 2.0), (8, 64, 2, 5.0, 2 * math.pi)):
+        F = np.array([rng.normal() for _ in range(nx * ny * nz)]).reshape(nx, ny, nz)
+        inp = os.path.join(TMP, "d_in.bin")
+        outp = os.path.join(TMP, "d_out.bin")
+        F.tofile(inp)
+        run([REF, "fderiv", str(nx), str(ny), str(nz), repr(lx), repr(ly), inp, outp])
+        r = np.fromfile(outp).reshape(4, nx, ny, nz)
+        out[f"{nx}x{ny}x{nz}"] = {"shape": [nx, ny, nz], "L": [lx, ly], "sha_input": sha(F),
+                                  "sha_out": [sha(r[w]) for w in range(4)],
+                                  "out": [hexs(r[w].ravel()) for w in range(4)] if F.size <= 4096 else None,
+                                  "head": [hexs(r[w].ravel()[:64]) for w in range(4)],
+                                  "max_abs": [float(np.abs(r[w]).max()) for w in range(4)]}
     return out
 
 
@@ -241,6 +261,10 @@ def config4_inputs():
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if "--fderiv" in sys.argv:
+        with open(os.path.join(HERE, "fderiv.json"), "w") as fp:
+            json.dump(fderiv_fixture(), fp, indent=0)
+        return
     if "--periodic" in sys.argv:
         for size in (1, 4):
             d, f, ex = periodic_shipped_inputs(size)

@@ -30,6 +30,9 @@ def test_shipped_problem_through_dropin(cuda):
     assert err <= 2.0e-13, err           # reference: 1.0891288e-13
     # d2/dz2 of the computed solution vs -(4 pi)^2 u_exact (spectral accuracy, N = 10)
     assert derr <= 1e-4 * 16 * 3.14159 ** 2, derr
+    # d2/dx2 (Fourier, exact for sin(8 pi x) on 64 points) vs -(8 pi)^2 u_exact
+    xerr = float(re.search(r"SecondDerivativeX error = (\S+)", r.stdout).group(1))
+    assert xerr <= 1e-9 * 64 * 3.14159 ** 2, xerr
 
 
 def test_callback_with_general_operator(cuda):

@@ -236,3 +236,29 @@ def test_periodic_collocation_residual():
     scale = max(np.abs(d).max() for d in dl)
     for e in range(ne):
         assert np.abs(dr[e] - dl[(e + 1) % ne]).max() < 1e-9 * scale
+
+
+def test_fderiv_fixture_convention():
+    """The reference's Fourier derivatives (Fourier_Quadrature.cpp:1494-1800)
+    are the spectral derivative with wavenumber j for j < L/2 and j - L
+    otherwise (the Nyquist point as -L/2): checked on its own output."""
+    from mpi_amd import workloads as wl
+    from golden_util import unhex
+    fx = load("fderiv.json")
+    rng = wl.SplitMix64(91)
+    for key in ["16x8x3", "64x32x5", "8x64x2"]:
+        rec = fx[key]
+        nx, ny, nz = rec["shape"]
+        F = np.array([rng.normal() for _ in range(nx * ny * nz)]).reshape(nx, ny, nz)
+        assert sha(F) == rec["sha_input"]
+        for w, (axis, order) in enumerate([(0, 1), (1, 1), (0, 2), (1, 2)]):
+            Lp = (nx, ny)[axis]
+            k = np.arange(Lp, dtype=np.float64)
+            k = np.where(k < Lp // 2, k, k - Lp)
+            c = 2 * math.pi * k / rec["L"][axis]
+            shape = [1, 1, 1]
+            shape[axis] = Lp
+            mult = (1j * c) ** order
+            want = np.fft.ifft(np.fft.fft(F, axis=axis) * mult.reshape(shape), axis=axis).real
+            got = unhex(rec["out"][w] if rec["out"] else rec["head"][w])
+            assert np.abs(got - want.ravel()[:len(got)]).max() <= 1e-12 * rec["max_abs"][w]
