@@ -67,7 +67,11 @@ enum {
 enum { HZ_FORM_PLAIN_LU = 0, HZ_FORM_DIAG = 1, HZ_FORM_MIXED = 2 };
 
 typedef struct {
-    int nx, ny;              /* Fourier points in x, y: powers of two >= 2 */
+    int nx, ny;              /* Fourier points in x, y: powers of two >= 1.
+                                nx = 1 is the 2-D solver of
+                                NonPeriodic2D_MPI.cpp (its periodic x is
+                                this y, its GLL y this z); nx = ny = 1 the
+                                1-D solver of NonPeriodic1D_MPI.cpp       */
     int n;                   /* GLL points per element (reference N), >= 3 */
     int ne;                  /* elements in z (reference size*Ns), >= 1    */
     double xl, xr, yl, yr;   /* periodic box                                */

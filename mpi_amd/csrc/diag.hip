@@ -45,9 +45,10 @@ int diag_padded(int n)
 
 int diag_tile_choices(int NP, int *rt, int max)
 {
-    static const int c64[] = {32, 64}, c128[] = {32, 48, 64}, c256[] = {16, 32};
+    // 128 rows (N-2 <= 64 only) holds a column of up to 64 elements
+    static const int c64[] = {32, 64, 128}, c128[] = {32, 48, 64}, c256[] = {16, 32};
     const int *c = NP == 64 ? c64 : NP == 128 ? c128 : c256;
-    const int k = NP == 128 ? 3 : 2;
+    const int k = NP == 256 ? 2 : 3;
     for (int i = 0; i < k && i < max; ++i) rt[i] = c[i];
     return k;
 }
@@ -56,7 +57,7 @@ int launch_diag_modes(const DiagArgs &a, hipStream_t s)
 {
     switch (diag_padded(a.n)) {
     case 64:
-        return a.rt == 32 ? launch_t<4, 32>(a, s) : launch_t<4, 64>(a, s);
+        return a.rt == 32 ? launch_t<4, 32>(a, s) : a.rt == 128 ? launch_t<4, 128>(a, s) : launch_t<4, 64>(a, s);
     case 128:
         return a.rt == 32 ? launch_t<8, 32>(a, s) : a.rt == 48 ? launch_t<8, 48>(a, s) : launch_t<8, 64>(a, s);
     case 256:

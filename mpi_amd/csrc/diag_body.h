@@ -183,15 +183,12 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     // row (end-point derivatives of the particular solution), ADJ needs
     // w0|wN . q (the adjoint load on zl, zr).  Each wave reduces its columns
     // over the 16 lanes of a row, then the four waves' partials are summed.
+    // One row block at a time, so that only its four partial pairs are live.
     double inv[RBK][JT][4];
-    double p0[RBK][4], pN[RBK][4];
     const double *dv0 = ADJ ? a.w0 : a.r0, *dvN = ADJ ? a.wN : a.rN;
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) p0[rb][r] = pN[rb][r] = 0.0;
-#pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
+    for (int rb = 0; rb < RBK; ++rb) {
+        double p0[4] = {0.0, 0.0, 0.0, 0.0}, pN[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (w * JT + jt) + li;
@@ -206,27 +203,28 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                 inv[rb][jt][r] = iv;
                 if (GENERAL || ADJ) {
                     const double q = acc[rb][jt][r] * iv;
-                    p0[rb][r] = fma(v0, q, p0[rb][r]);
-                    pN[rb][r] = fma(vN, q, pN[rb][r]);
+                    p0[r] = fma(v0, q, p0[r]);
+                    pN[r] = fma(vN, q, pN[r]);
                 }
             }
         }
-    if constexpr (GENERAL || ADJ) {
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb)
+        if constexpr (GENERAL || ADJ) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
 #pragma unroll
                 for (int o = 1; o < 16; o <<= 1) {
-                    p0[rb][r] += __shfl_xor(p0[rb][r], o);
-                    pN[rb][r] += __shfl_xor(pN[rb][r], o);
+                    p0[r] += __shfl_xor(p0[r], o);
+                    pN[r] += __shfl_xor(pN[r], o);
                 }
                 if (li == 0) {
                     const int row = 16 * rb + 4 * r + lk;
-                    part0[w][row] = p0[rb][r];
-                    partN[w][row] = pN[rb][r];
+                    part0[w][row] = p0[r];
+                    partN[w][row] = pN[r];
                 }
             }
+        }
+    }
+    if constexpr (GENERAL || ADJ) {
         __syncthreads();
         for (int row = tid; row < RT; row += 256) {
             double s0 = 0.0, sN = 0.0;

@@ -53,6 +53,61 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
     return HZ_OK;
 }
 
+// Length-1 lines (an axis of one point: the 2-D and 1-D problems): the
+// transform of one point is the point itself (1/L = 1 on the inverse), so
+// each lane is copied; R2C writes a zero imaginary part, C2R keeps the real
+// part.  One thread per (batch, lane), lanes fastest.
+template <int KIND>
+__global__ __launch_bounds__(256) void fft_unit_lines(const FftArgs a)
+{
+    const int nl = a.lanes + a.xlanes;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nl * a.nbatch) return;
+    const int b = (int)(t / nl), l = (int)(t % nl);
+    const double *ire, *iim = nullptr;
+    double *ore, *oim = nullptr;
+    if (l < a.lanes) {
+        const long long io = (long long)b * a.in_batch_stride + l, oo = (long long)b * a.out_batch_stride + l;
+        ire = a.in_re + io;
+        if (KIND != FFT_R2C) iim = a.in_im + io;
+        ore = a.out_re + oo;
+        if (KIND != FFT_C2R) oim = a.out_im + oo;
+    } else {
+        const int sx = l - a.lanes;
+        const long long io = (long long)b * a.x_in_batch_stride, oo = (long long)b * a.x_out_batch_stride;
+        ire = a.xin_re[sx] + io;
+        if (KIND != FFT_R2C) iim = a.xin_im[sx] + io;
+        ore = a.xout_re[sx] + oo;
+        if (KIND != FFT_C2R) oim = a.xout_im[sx] + oo;
+    }
+    const double re = *ire;
+    const double im = (KIND == FFT_C2C_FWD || KIND == FFT_C2C_INV) ? *iim : 0.0;
+    *ore = re;
+    if (KIND != FFT_C2R) *oim = im;
+}
+
+int launch_unit(FftKind kind, const FftArgs &a, hipStream_t s)
+{
+    // an in-place complex pass over one point changes nothing
+    const bool inplace = a.in_re == a.out_re && a.in_im == a.out_im &&
+                         (a.xlanes == 0 || (a.xin_re[0] == a.xout_re[0] && a.xin_im[0] == a.xout_im[0] &&
+                                            (a.xlanes < 2 || (a.xin_re[1] == a.xout_re[1] &&
+                                                              a.xin_im[1] == a.xout_im[1]))));
+    if ((kind == FFT_C2C_FWD || kind == FFT_C2C_INV) && inplace &&
+        a.in_batch_stride == a.out_batch_stride && a.x_in_batch_stride == a.x_out_batch_stride)
+        return HZ_OK;
+    const long long total = (long long)(a.lanes + a.xlanes) * a.nbatch;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    switch (kind) {
+    case FFT_R2C: hipLaunchKernelGGL((fft_unit_lines<FFT_R2C>), grid, block, 0, s, a); break;
+    case FFT_C2R: hipLaunchKernelGGL((fft_unit_lines<FFT_C2R>), grid, block, 0, s, a); break;
+    case FFT_C2C_FWD: hipLaunchKernelGGL((fft_unit_lines<FFT_C2C_FWD>), grid, block, 0, s, a); break;
+    case FFT_C2C_INV: hipLaunchKernelGGL((fft_unit_lines<FFT_C2C_INV>), grid, block, 0, s, a); break;
+    }
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
 }  // namespace
 
 int fft_width_choices(int L, int *w, int max)
@@ -67,6 +122,7 @@ int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
 {
     if (a.nbatch <= 0 || a.lanes + a.xlanes <= 0) return HZ_OK;
     switch (L) {
+    case 1: return launch_unit(kind, a, s);
     case 2: return launch_kind<2, 2, 1, 16>(kind, a, s);
     case 4: return launch_kind<4, 4, 1, 16>(kind, a, s);
     case 8: return launch_kind<8, 8, 1, 16>(kind, a, s);
@@ -83,7 +139,7 @@ int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
                : a.w == 32 ? launch_kind<256, 16, 16, 32>(kind, a, s) : launch_kind<256, 16, 16, 16>(kind, a, s);
     case 512: return launch_kind<512, 32, 16, 8>(kind, a, s);
     case 1024: return launch_kind<1024, 32, 32, 4>(kind, a, s);
-    default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [2, 1024]");
+    default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [1, 1024]");
     }
 }
 

@@ -81,6 +81,9 @@ struct LuModeArgs {
     int periodic;            // z periodic: cyclic interface system, no face data
 };
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s);
+// Whether the plain-LU mode kernel has an instance for (nx, N-2, ne); sets
+// the error text when not.
+int lu_modes_check(int nx, int n, int ne);
 
 // Arguments of the z-diagonalised mode kernel (diag.hip).
 struct DiagArgs {

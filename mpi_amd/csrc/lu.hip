@@ -41,6 +41,13 @@ namespace {
 
 constexpr double kEps = 1.000000000000E-12;  // include/General.h:15
 constexpr int WAVE = 64;
+// Right-hand sides: ul, ur, then per element one per real column of the
+// mode (2 per complex column; 2 complex columns when 0 < |kx| < nx/2, which
+// needs nx >= 4).  At most 4 x 64 (the NR = 64 instance); the LDS arrays
+// indexed by right-hand side are sized with room to spare.
+constexpr int kMaxRhs = 2 + 4 * 64;
+
+__host__ __device__ __forceinline__ int rhs_per_element(int nx) { return nx >= 4 ? 4 : 2; }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -312,8 +319,8 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     __shared__ Smem<RB, CB, NR, TC> sm;
     __shared__ double ubuf[2][BQ][ROWS];
     __shared__ double sh_ul[ROWS], sh_ur[ROWS];
-    __shared__ double dots[2][128];
-    __shared__ double zeta[2][128];
+    __shared__ double dots[2][kMaxRhs];
+    __shared__ double zeta[2][kMaxRhs];
 
     const int lane = threadIdx.x % WAVE, c = uniform(threadIdx.x / WAVE);
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
@@ -321,7 +328,8 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     const int iw = mode / M, jw = mode % M;
     const int xi0 = iw, xi1 = a.nx - iw, nxi = (iw != 0 && iw != a.nx / 2) ? 2 : 1;
     const int nc = 2 * nxi;
-    const int nrhs = 2 + 4 * E;
+    const int CPE = rhs_per_element(a.nx);  // right-hand sides per element
+    const int nrhs = 2 + CPE * E;
 
     // ---- assemble (physical row i = position i) ----
     double A[RB][CB], Y[RB][NR];
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
                 } else if (r == 1) {
                     v = -a.Dzz[(long long)(i + 1) * N + (N - 1)];
                 } else {
-                    const int e = (r - 2) >> 2, cc = (r - 2) & 3;
+                    const int e = (r - 2) / CPE, cc = (r - 2) % CPE;
                     if (cc < nc) {
                         const int xi = (cc >> 1) ? xi1 : xi0;
                         const long long col = ((long long)xi * M + jw) * K;
@@ -443,7 +451,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
             d0 += a.Dz[N - 1];
             dN += a.Dz[(long long)(N - 1) * N + N - 1];
         }
-        if (lane == 0 && r < 128) {
+        if (lane == 0) {
             dots[0][r] = d0;
             dots[1][r] = dN;
         }
@@ -454,7 +462,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     if (threadIdx.x < nc && a.periodic) {
         const int cc = threadIdx.x;
         periodic_sweep(E, dots[0][0], dots[1][0], dots[0][1], dots[1][1], &dots[0][2 + cc], &dots[1][2 + cc],
-                       &zeta[0][cc], &zeta[1][cc], 4);
+                       &zeta[0][cc], &zeta[1][cc], CPE);
     } else if (threadIdx.x < nc) {
         const int cc = threadIdx.x;
         const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
@@ -470,23 +478,23 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
         for (int e = 1; e < E; ++e) {
             const double lb = dNul;
             const double be = dNur - d0ul - (cprev / bprev) * lb;
-            const double re = dots[0][2 + 4 * e + cc] - dots[1][2 + 4 * (e - 1) + cc] - (rprev / bprev) * lb;
+            const double re = dots[0][2 + CPE * e + cc] - dots[1][2 + CPE * (e - 1) + cc] - (rprev / bprev) * lb;
             bprev = be;
             cprev = -d0ur;
             rprev = re;
-            zeta[0][4 * e + cc] = be;
-            zeta[1][4 * e + cc] = re;
+            zeta[0][CPE * e + cc] = be;
+            zeta[1][CPE * e + cc] = re;
         }
         const double lb = a.alpha_r * dNul;
         const double br = a.alpha_r * dNur + a.beta_r - (cprev / bprev) * lb;
-        const double rhsr = gr - a.alpha_r * dots[1][2 + 4 * (E - 1) + cc] - (rprev / bprev) * lb;
+        const double rhsr = gr - a.alpha_r * dots[1][2 + CPE * (E - 1) + cc] - (rprev / bprev) * lb;
         double zr = rhsr / br;
         for (int e = E - 1; e >= 0; --e) {
-            const double be = zeta[0][4 * e + cc], re = zeta[1][4 * e + cc];
+            const double be = zeta[0][CPE * e + cc], re = zeta[1][CPE * e + cc];
             const double ce = (e == 0) ? -a.alpha_l * d0ur : -d0ur;
             const double zl = (re - ce * zr) / be;
-            zeta[0][4 * e + cc] = zl;
-            zeta[1][4 * e + cc] = zr;
+            zeta[0][CPE * e + cc] = zl;
+            zeta[1][CPE * e + cc] = zr;
             zr = zl;
         }
     }
@@ -497,9 +505,9 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     for (int t = 0; t < NR; ++t) {
         const int r = c + TC * t;
         if (r < 2 || r >= nrhs) continue;
-        const int e = (r - 2) >> 2, cc = (r - 2) & 3;
+        const int e = (r - 2) / CPE, cc = (r - 2) % CPE;
         if (cc >= nc) continue;
-        const double zl = zeta[0][4 * e + cc], zr = zeta[1][4 * e + cc];
+        const double zl = zeta[0][CPE * e + cc], zr = zeta[1][CPE * e + cc];
         const int xi = (cc >> 1) ? xi1 : xi0;
         const long long col = ((long long)xi * M + jw) * K + (long long)e * (N - 1);
         double *dst = (cc & 1) ? a.spec_im : a.spec_re;
@@ -594,25 +602,36 @@ int launch_factor_t(const LuFactorArgs &a, hipStream_t s)
 template <int RB, int CB>
 int launch_modes_nr(const LuModeArgs &a, int modes, hipStream_t s)
 {
-    const int nr = (2 + 4 * a.ne + 3) / 4;  // right-hand sides per wave
+    const int nr = (2 + rhs_per_element(a.nx) * a.ne + 3) / 4;  // right-hand sides per wave
     if (nr <= 2) return launch_modes_t<RB, CB, 2>(a, modes, s);
     if (nr <= 8) return launch_modes_t<RB, CB, 8>(a, modes, s);
     if constexpr (RB == 1) {
         if (nr <= 32) return launch_modes_t<1, CB, 32>(a, modes, s);
+        if (nr <= 64) return launch_modes_t<1, CB, 64>(a, modes, s);
     }
-    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: at most 7 elements when N-2 > 64");
+    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: too many elements (2 + c*ne right-hand sides, c = 4 "
+                                         "when nx >= 4 else 2, must be <= 256, or <= 32 when N-2 > 64)");
 }
 
 }  // namespace
 
+int lu_modes_check(int nx, int n, int ne)
+{
+    if (n > 128) return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: N-2 must be <= 128");
+    const int nrhs = 2 + rhs_per_element(nx) * ne;
+    if (nrhs > 4 * (n <= 64 ? 64 : 8))  // 4 waves x NR of the largest instance
+        return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: too many elements (at most 63, or 127 when nx < 4; "
+                                             "7, or 15 when nx < 4, for N-2 > 64)");
+    return HZ_OK;
+}
+
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s)
 {
-    if (a.ne > 31) return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form supports at most 31 elements");
+    if (int st = lu_modes_check(a.nx, a.n, a.ne)) return st;
     if (a.n <= 16) return launch_modes_nr<1, 4>(a, modes, s);
     if (a.n <= 32) return launch_modes_nr<1, 8>(a, modes, s);
     if (a.n <= 64) return launch_modes_nr<1, 16>(a, modes, s);
-    if (a.n <= 128) return launch_modes_nr<2, 32>(a, modes, s);
-    return set_error(HZ_ERR_UNSUPPORTED, "PLAIN_LU form: N-2 must be <= 128");
+    return launch_modes_nr<2, 32>(a, modes, s);
 }
 
 int launch_lu_factor(const LuFactorArgs &a, hipStream_t s)

@@ -31,7 +31,7 @@ using namespace hz;
 
 namespace {
 
-bool is_pow2(int v) { return v >= 2 && (v & (v - 1)) == 0; }
+bool is_pow2(int v) { return v >= 1 && (v & (v - 1)) == 0; }
 
 template <class T>
 int dmalloc(T **p, size_t count)
@@ -464,6 +464,9 @@ int tune_diag(hz_plan *p)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
+    if (!st && best == 1e300)
+        st = set_error(HZ_ERR_UNSUPPORTED, "DIAG form: 2*ne must fit one tile (ne <= 64 when N-2 <= 64, "
+                                           "32 when N-2 <= 128, 16 when N-2 <= 256)");
     return st;
 }
 
@@ -614,7 +617,8 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
-    if ((st = tune_fft(p)) || (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
+    if ((d.form == HZ_FORM_PLAIN_LU && (st = lu_modes_check(p->nx, p->n, p->ne))) || (st = tune_fft(p)) ||
+        (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
         free_plan(p);
         return st;
     }

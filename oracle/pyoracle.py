@@ -12,7 +12,12 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libhz_oracle.so")
+# The same C source built with fused multiply-adds (oracle/Makefile): the
+# reference's algorithm under a different rounding, used only to measure how
+# far rounding alone moves a result (the parity bar of ill-conditioned cases).
+_FMA_PATH = os.path.join(_HERE, "libhz_oracle_fma.so")
 _lib = None
+_fma = None
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -37,12 +42,13 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(fma=False):
+    global _lib, _fma
+    if (_fma if fma else _lib) is None:
+        path = _FMA_PATH if fma else _LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         L.hzo_error_string.restype = ctypes.c_char_p
         L.hzo_legendre.restype = ctypes.c_double
         L.hzo_legendre.argtypes = [ctypes.c_double, ctypes.c_int, _I]
@@ -63,8 +69,11 @@ def lib():
         L.hzo_solve.argtypes = [ctypes.POINTER(Desc), _D, _D, _D, _D]
         L.hzo_derivative_z.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, _D, _D]
         L.hzo_solve_periodic.argtypes = [ctypes.POINTER(Desc), _D, _D]
-        _lib = L
-    return _lib
+        if fma:
+            _fma = L
+        else:
+            _lib = L
+    return _fma if fma else _lib
 
 
 def _p(a):
@@ -145,23 +154,24 @@ def z_grid(desc):
     return z
 
 
-def solve(desc, f, gl, gr):
-    """Full reference SolvePDE with desc.size ranks simulated in-process."""
+def solve(desc, f, gl, gr, fma=False):
+    """Full reference SolvePDE with desc.size ranks simulated in-process
+    (fma: the fused-multiply-add build, for rounding-sensitivity checks)."""
     K = total_z_points(desc)
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(desc.Nx, desc.Ny, K)
     gl = np.ascontiguousarray(gl, dtype=np.float64).reshape(desc.Nx, desc.Ny)
     gr = np.ascontiguousarray(gr, dtype=np.float64).reshape(desc.Nx, desc.Ny)
     u = np.zeros_like(f)
-    _check(lib().hzo_solve(ctypes.byref(desc), _p(f), _p(gl), _p(gr), _p(u)))
+    _check(lib(fma).hzo_solve(ctypes.byref(desc), _p(f), _p(gl), _p(gr), _p(u)))
     return u
 
 
-def solve_periodic(desc, f):
+def solve_periodic(desc, f, fma=False):
     """Reference periodic-z SolvePDE (Periodic3D_MPI.cpp) at size = 1."""
     K = total_z_points(desc)
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(desc.Nx, desc.Ny, K)
     u = np.zeros_like(f)
-    _check(lib().hzo_solve_periodic(ctypes.byref(desc), _p(f), _p(u)))
+    _check(lib(fma).hzo_solve_periodic(ctypes.byref(desc), _p(f), _p(u)))
     return u
 
 

@@ -12,6 +12,7 @@ plus sampled values and norms (large cases).
     python tests/golden/make_golden.py --config4     # 512 x 512 x 257 (minutes)
     python tests/golden/make_golden.py --periodic    # the periodic-z solver
     python tests/golden/make_golden.py --fderiv      # Fourier derivatives
+    python tests/golden/make_golden.py --lowdim      # the 2-D and 1-D solvers
 """
 import hashlib
 import json
@@ -161,6 +162,130 @@ def ref_solve_periodic(desc, f):
     return u, float(res.stdout.split()[1])
 
 
+LOWREF = {(2, False): "ref_driver_2d", (1, False): "ref_driver_1d",
+          (2, True): "ref_driver_p2d", (1, True): "ref_driver_p1d"}
+
+
+def ref_solve_low(dim, periodic, p, grid_only=False, f=None, gl=None, gr=None):
+    """NonPeriodic/Periodic 2D/1D_MPI SolvePDE under mpiexec -np p['size'].
+    Returns (u, grid, seconds); u is [Nx][K] (2-D) or [K] (1-D)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", LOWREF[(dim, periodic)])
+    N, size = p["N"], p["size"]
+    along = p["Ny"] if dim == 2 else p["Nx"]
+    Ns = -(-along // (size * N))
+    K = size * Ns * (N - 1) + 1
+    lines = p["Nx"] if dim == 2 else 1
+    inp = os.path.join(TMP, "l_in.bin")
+    outp = os.path.join(TMP, "l_out.bin")
+    if grid_only:
+        f = np.zeros((lines, K))
+        gl = gr = np.zeros(lines)
+    parts = [np.asarray(f, dtype=np.float64).ravel()]
+    if not periodic:
+        parts += [np.atleast_1d(gl).astype(np.float64).ravel(), np.atleast_1d(gr).astype(np.float64).ravel()]
+    np.concatenate(parts).tofile(inp)
+    args = [MPIEXEC, "-np", str(size), exe, "solve", str(p["Nx"])]
+    if dim == 2:
+        args += [str(p["Ny"])]
+    args += [str(N), repr(p["xl"]), repr(p["xr"])]
+    if dim == 2:
+        args += [repr(p["yl"]), repr(p["yr"])]
+    args += [repr(p[k]) for k in ("al", "bl", "ar", "br", "lam")] + [inp, outp]
+    res = run(args)
+    u = np.zeros((lines, K))
+    grid = np.zeros(K)
+    for r in range(size):
+        a = np.fromfile(f"{outp}.rank{r}")
+        blk = a[: Ns * lines * N].reshape(Ns, lines, N)
+        loc = a[Ns * lines * N:]
+        for kk in range(Ns):
+            gidx = (r * Ns + kk) * (N - 1)
+            u[:, gidx:gidx + N] = blk[kk]
+            grid[gidx:gidx + N] = loc[kk * (N - 1):kk * (N - 1) + N]
+    if dim == 1:
+        u = u[0]
+    return u, grid, float(res.stdout.split()[1])
+
+
+def lowdim_case(name, dim, periodic, p, gen):
+    """One 2-D / 1-D golden case: the reference's grid first (a zero solve),
+    then the inputs regenerated by tests/golden_util.lowdim_inputs and the
+    reference's solution of them."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import golden_util as gu
+    _, grid, _ = ref_solve_low(dim, periodic, p, grid_only=True)
+    K = grid.size
+    rec = {"dim": dim, "periodic": periodic, "params": p, "gen": gen, "K": K,
+           "elements": (K - 1) // (p["N"] - 1), "grid": hexs(grid)}
+    # inputs first with placeholder hashes, then the real ones
+    rec["sha_f"] = rec["sha_gl"] = rec["sha_gr"] = None
+    gu_sha, gu.sha = gu.sha, (lambda a: None)
+    try:
+        f, gl, gr, ex = gu.lowdim_inputs(rec)
+    finally:
+        gu.sha = gu_sha
+    rec["sha_f"] = sha(f)
+    if gl is not None:
+        rec["sha_gl"], rec["sha_gr"] = sha(gl), sha(gr)
+    u, grid2, secs = ref_solve_low(dim, periodic, p, f=f, gl=gl, gr=gr)
+    assert np.array_equal(grid, grid2)
+    rec.update({"sha_u": sha(u), "max_abs_u": float(np.abs(u).max()), "ref_seconds": secs})
+    # How far rounding alone moves the reference's algorithm on this case:
+    # the oracle built with fused multiply-adds against the reference's output.
+    from test_oracle_lowdim import as3d, oracle_desc
+    d = oracle_desc(rec)
+    F, GL, GR = as3d(rec, f, gl, gr)
+    uf = (po.solve_periodic(d, F, fma=True) if periodic else po.solve(d, F, GL, GR, fma=True)).reshape(u.shape)
+    rec["rounding_spread"] = float(np.abs(uf - u).max() / np.abs(u).max())
+    if ex is not None:
+        rec["err_vs_exact"] = float(np.abs(u - ex).max())
+    if u.size <= 20000:
+        rec["u"] = hexs(u)
+    else:
+        rng = wl.SplitMix64(99)
+        idx = [[rng.randint(0, s - 1) for s in u.shape] for _ in range(4096)]
+        rec["sample_idx"] = idx
+        rec["sample_u"] = hexs([u[i, k] for i, k in idx])
+    with open(os.path.join(HERE, f"low_{name}.json"), "w") as fp:
+        json.dump(rec, fp, indent=0)
+    print(f"{name}: K {K} elements {rec['elements']} ref {secs:.3f}s err_vs_exact {rec.get('err_vs_exact')} "
+          f"rounding_spread {rec['rounding_spread']:.2e}")
+
+
+def lowdim_fixtures():
+    two = dict(xl=0.0, xr=1.0, yl=0.0, yr=1.0)
+    robin_a = dict(al=1.0, bl=0.0, ar=0.0, br=1.0)
+    none = dict(al=0.0, bl=1.0, ar=0.0, br=1.0)
+    shipped = {"kind": "shipped"}
+    for size in (1, 4):  # TestNonPeriodic2D_MPI.cpp:84-118
+        lowdim_case(f"2d_shipped_np{size}", 2, False, dict(Nx=512, Ny=512, N=10, **two, **robin_a, lam=1.0, size=size),
+                    shipped)
+    lowdim_case("p2d_shipped_np1", 2, True, dict(Nx=512, Ny=512, N=10, **two, **none, lam=1.0, size=1), shipped)
+    tau = 2 * math.pi
+    lowdim_case("2d_random_64_N17", 2, False,
+                dict(Nx=64, Ny=51, N=17, xl=0.0, xr=tau, yl=-1.0, yr=1.0, al=0.5, bl=1.0, ar=0.25, br=2.0, lam=10.0,
+                     size=1), {"kind": "random", "seed": 11, "kmax": 16, "lmax": 16, "zl": -1.0})
+    lowdim_case("2d_random_128_N33", 2, False,
+                dict(Nx=128, Ny=33, N=33, xl=0.0, xr=tau, yl=-1.0, yr=1.0, **none, lam=5.0, size=1),
+                {"kind": "random", "seed": 12, "kmax": 32, "lmax": 32, "zl": -1.0})
+    lowdim_case("p2d_random_64_N17", 2, True,
+                dict(Nx=64, Ny=51, N=17, xl=0.0, xr=tau, yl=-1.0, yr=1.0, **none, lam=3.0, size=1),
+                {"kind": "random", "seed": 13, "kmax": 16, "lmax": 16, "zl": -1.0})
+    one = dict(xl=0.0, xr=1.0)
+    for size in (1, 4):  # TestNonPeriodic1D_MPI.cpp:91-118, first problem
+        lowdim_case(f"1d_shipped_a_np{size}", 1, False, dict(Nx=512, N=8, **one, **robin_a, lam=1.0, size=size),
+                    shipped)
+    lowdim_case("1d_shipped_b_np1", 1, False,  # :136-149, second problem
+                dict(Nx=512, N=8, **one, al=1.0, bl=1.0, ar=0.0, br=1.0, lam=5.0, size=1), shipped)
+    lowdim_case("p1d_shipped_np1", 1, True, dict(Nx=512, N=8, **one, **none, lam=1.0, size=1), shipped)
+    lowdim_case("1d_random_N33", 1, False,
+                dict(Nx=66, N=33, xl=-1.0, xr=1.0, al=0.5, bl=1.0, ar=0.25, br=2.0, lam=10.0, size=1),
+                {"kind": "random", "seed": 14, "kmax": 0, "lmax": 32, "zl": -1.0})
+    lowdim_case("p1d_random_N17", 1, True,
+                dict(Nx=68, N=17, xl=-1.0, xr=1.0, **none, lam=2.0, size=1),
+                {"kind": "random", "seed": 15, "kmax": 0, "lmax": 16, "zl": -1.0})
+
+
 def sample_idx(shape, count, seed):
     rng = wl.SplitMix64(seed)
     return [[rng.randint(0, s - 1) for s in shape] for _ in range(count)]
@@ -261,6 +386,9 @@ def config4_inputs():
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    if "--lowdim" in sys.argv:
+        lowdim_fixtures()
+        return
     if "--fderiv" in sys.argv:
         with open(os.path.join(HERE, "fderiv.json"), "w") as fp:
             json.dump(fderiv_fixture(), fp, indent=0)

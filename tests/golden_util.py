@@ -83,3 +83,84 @@ def periodic_case_inputs(name, z):
     f = np.ascontiguousarray(f)
     assert sha(f) == rec["sha_f"], "input generator drifted from the golden vectors"
     return rec, f, ex
+
+
+# ---------------------------------------------------------------------------
+# 2-D and 1-D solvers (NonPeriodic2D/1D_MPI.cpp, Periodic2D/1D_MPI.cpp)
+
+def lowdim_inputs(rec):
+    """Regenerate the inputs of a 2-D / 1-D golden case from its record.
+
+    Returns (f, gl, gr, exact): 2-D arrays are [Nx][K] (the reference's
+    periodic x, then its GLL axis), 1-D ones [K]; faces are [Nx] / scalars
+    (None for the periodic solvers); exact is None for random cases."""
+    from mpi_amd import workloads as wl
+    PI = 3.1415926535897932
+    p, g = rec["params"], rec["gen"]
+    grid = unhex(rec["grid"])
+    dim, periodic = rec["dim"], rec["periodic"]
+    xl, xr = p["xl"], p["xr"]
+    Lx = xr - xl
+    lam = p["lam"]
+    al, bl, ar, br = p["al"], p["bl"], p["ar"], p["br"]
+    gl = gr = ex = None
+    if g["kind"] == "shipped" and dim == 2:
+        yl, yr = p["yl"], p["yr"]
+        Ly = yr - yl
+        x = xl + np.arange(p["Nx"]) * (Lx / p["Nx"])
+        X, Y = np.meshgrid(x, grid, indexing="ij")
+        sx = np.sin(8.0 * PI * (X - xl) / Lx)
+        f = -(64.0 * PI * PI / (Lx * Lx) + 36.0 * PI * PI / (Ly * Ly) + lam) * sx * np.cos(6.0 * PI * (Y - yl) / Ly)
+        ex = sx * np.cos(6.0 * PI * (Y - yl) / Ly)
+        if not periodic:
+            s = np.sin(8.0 * PI * (x - xl) / Lx)
+            gl = (al * 6.0 * PI * np.sin(6.0 * PI * (yl - yl) / Ly) / Ly + bl * np.cos(6.0 * PI * (yl - yl) / Ly)) * s
+            gr = (-ar * 6.0 * PI * np.sin(6.0 * PI * (yr - yl) / Ly) / Ly + br * np.cos(6.0 * PI * (yr - yl) / Ly)) * s
+    elif g["kind"] == "shipped":
+        if periodic:
+            f = -(64.0 * PI * PI / (Lx * Lx) + lam) * np.sin(8.0 * PI * (grid - xl) / Lx)
+            ex = np.sin(8.0 * PI * (grid - xl) / Lx)
+        else:
+            f = -(4.0 * PI * PI / (Lx * Lx) + lam) * np.sin(2.0 * PI * (grid - xl) / Lx)
+            ex = np.sin(2.0 * PI * (grid - xl) / Lx)
+            gl = np.float64(-al * 2.0 * PI * math.cos(2.0 * PI * (xl - xl) / Lx) / Lx + bl * math.sin(0.0))
+            gr = np.float64(ar * 2.0 * PI * math.cos(2.0 * PI * (xr - xl) / Lx) / Lx +
+                            br * math.sin(2.0 * PI * (xr - xl) / Lx))
+    else:  # random smooth modes; the GLL axis mapped to [-1, 1]
+        zeta = grid + g["zl"]
+        if dim == 2:
+            x = xl + np.arange(p["Nx"]) * (Lx / p["Nx"])
+            f3, gl3, gr3 = wl.random_problem(np.array([0.0]), x, zeta, seed=g["seed"], nmodes=16,
+                                             kmax=g["kmax"], lmax=g["lmax"])
+            f, gl, gr = f3[0], gl3[0], gr3[0]
+        else:
+            f3, gl3, gr3 = wl.random_problem(np.array([0.0]), np.array([0.0]), zeta, seed=g["seed"], nmodes=16,
+                                             kmax=0, lmax=g["lmax"])
+            f, gl, gr = f3[0, 0], gl3[0, 0], gr3[0, 0]
+        if periodic:
+            gl = gr = None
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if gl is not None:
+        gl = np.ascontiguousarray(gl, dtype=np.float64)
+        gr = np.ascontiguousarray(gr, dtype=np.float64)
+    assert sha(f) == rec["sha_f"], "input generator drifted from the golden vectors"
+    if gl is not None:
+        assert sha(gl) == rec["sha_gl"] and sha(gr) == rec["sha_gr"], "face generator drifted"
+    return f, gl, gr, ex
+
+
+def lowdim_reference_u(rec):
+    """The reference's solution: full array when stored, else None."""
+    if "u" in rec:
+        shape = (rec["params"]["Nx"], rec["K"]) if rec["dim"] == 2 else (rec["K"],)
+        return unhex(rec["u"]).reshape(shape)
+    return None
+
+
+def lowdim_sample(u, rec):
+    """u at the record's sample points (2-D: (i, k) pairs) and the reference's values there."""
+    if "u" in rec:
+        ref = lowdim_reference_u(rec)
+        return np.asarray(u).ravel(), ref.ravel()
+    idx = rec["sample_idx"]
+    return np.array([u[i, k] for i, k in idx]), unhex(rec["sample_u"])
