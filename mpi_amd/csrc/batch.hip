@@ -480,6 +480,10 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         hipHostMalloc((void **)&b->h_probs, sizeof(BatchProblem) * count) != hipSuccess ||
         hipMalloc((void **)&b->d_ctl, sizeof(int) * b->nctl) != hipSuccess)
         return fail(set_error(HZ_ERR_ALLOC, "batch allocation failed"));
+    // the staged variants never write ctl; hz_batch_status reads its error
+    // flag whatever ran, so it starts cleared (outside any graph capture)
+    if (hipMemset(b->d_ctl, 0, sizeof(int) * b->nctl) != hipSuccess)
+        return fail(set_error(HZ_ERR_HIP, "batch control clear failed"));
     for (int r = 0; r < 2; ++r) {
         std::vector<std::array<int, kStages>> counts;
         b->h_tab[r] = task_table(b, r ? 64 : 32, &counts);
