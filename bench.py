@@ -64,6 +64,39 @@ def fft_bytes(nx, ny, K):
     return 2 * (real + half) + 2 * (2 * half)
 
 
+def whole_solve_roofline(form, ms_per_solve, nx, ny, n_gll, refine_sweeps=3):
+    """Whole-solve roofline: sum over the phases (forward transform, mode
+    solve, inverse transform) of max(flop / FP64 peak, bytes / HBM peak),
+    divided by the measured time per solve.  Bytes per transform are the
+    three-kernel figure (read the field, write the half spectrum, or the
+    reverse); `frac_executed` counts the two line passes per direction this
+    build runs instead."""
+    K = n_gll
+    real = nx * ny * K * 8
+    half = nx * (ny // 2 + 1) * K * 16
+    n = n_gll - 2
+    cols = nx * (ny // 2 + 1) * 2
+    if form == "plain":
+        mode_flop, mode_bytes = lu_flops(nx, ny, n_gll), 2 * half
+    elif form == "diag":
+        mode_flop, mode_bytes = diag_flops(nx, ny, n_gll), 2 * half
+    else:  # FP32 operator per mode read once, applied and refined per column
+        modes = (nx // 2 + 1) * (ny // 2 + 1)
+        mode_flop = cols * (refine_sweeps + 1) * 4 * n * n
+        mode_bytes = 2 * half + modes * n * n * 4
+    phases = {"fft_forward": (0.0, real + half), "mode_solve": (mode_flop, mode_bytes),
+              "fft_inverse": (0.0, half + real)}
+    ideal = {k: max(fl / PEAK_FP64, by / PEAK_HBM) * 1e3 for k, (fl, by) in phases.items()}
+    executed = dict(ideal)
+    # R2C (read real, write half) + C2C in place (read and write half); the reverse
+    executed["fft_forward"] = executed["fft_inverse"] = (real + 3 * half) / PEAK_HBM * 1e3
+    return {"ideal_ms": sum(ideal.values()), "measured_ms": ms_per_solve,
+            "frac": sum(ideal.values()) / ms_per_solve,
+            "frac_executed": sum(executed.values()) / ms_per_solve,
+            "phase_ideal_ms": ideal,
+            "note": "sum_phase max(flop/78.6 TFLOP/s, bytes/8 TB/s) / measured ms per solve"}
+
+
 STAGE_KERNELS = {"diag_modes": "diag_modes_kernel", "lu_modes": "lu_modes_kernel",
                  "fft_forward": "fft_lines", "fft_inverse": "fft_lines"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_256x256x129.json")
@@ -403,6 +436,7 @@ def main():
                   for k, v in results.items()},
         "chosen_vs_plain": (chosen["solves_per_s"] / results["plain"]["solves_per_s"]) if "plain" in results else None,
         "roofline": roof,
+        "whole_solve_roofline": whole_solve_roofline(chosen["form"], chosen["ms_per_solve"], NX, NY, N),
     }
     if not args.no_extras and "diag" in results:
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])

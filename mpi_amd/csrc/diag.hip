@@ -45,10 +45,11 @@ int diag_padded(int n)
 
 int diag_tile_choices(int NP, int *rt, int max)
 {
+    // 16 rows: more work-groups in flight to hide the staging latency;
     // 128 rows (N-2 <= 64 only) holds a column of up to 64 elements
-    static const int c64[] = {32, 64, 128}, c128[] = {32, 48, 64}, c256[] = {16, 32};
+    static const int c64[] = {16, 32, 64, 128}, c128[] = {16, 32, 48, 64}, c256[] = {16, 32};
     const int *c = NP == 64 ? c64 : NP == 128 ? c128 : c256;
-    const int k = NP == 256 ? 2 : 3;
+    const int k = NP == 256 ? 2 : 4;
     for (int i = 0; i < k && i < max; ++i) rt[i] = c[i];
     return k;
 }
@@ -57,9 +58,15 @@ int launch_diag_modes(const DiagArgs &a, hipStream_t s)
 {
     switch (diag_padded(a.n)) {
     case 64:
-        return a.rt == 32 ? launch_t<4, 32>(a, s) : a.rt == 128 ? launch_t<4, 128>(a, s) : launch_t<4, 64>(a, s);
+        return a.rt == 16   ? launch_t<4, 16>(a, s)
+               : a.rt == 32  ? launch_t<4, 32>(a, s)
+               : a.rt == 128 ? launch_t<4, 128>(a, s)
+                             : launch_t<4, 64>(a, s);
     case 128:
-        return a.rt == 32 ? launch_t<8, 32>(a, s) : a.rt == 48 ? launch_t<8, 48>(a, s) : launch_t<8, 64>(a, s);
+        return a.rt == 16  ? launch_t<8, 16>(a, s)
+               : a.rt == 32 ? launch_t<8, 32>(a, s)
+               : a.rt == 48 ? launch_t<8, 48>(a, s)
+                            : launch_t<8, 64>(a, s);
     case 256:
         return a.rt == 16 ? launch_t<16, 16>(a, s) : launch_t<16, 32>(a, s);
     default: return set_error(HZ_ERR_UNSUPPORTED, "DIAG form: N-2 must be <= 256");

@@ -21,33 +21,77 @@ namespace {
 
 constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
 
+// XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs, so
+// XCD x is given the contiguous logical range [x*T/8, (x+1)*T/8) of tiles
+// (lane block fastest).  Neighbouring lane blocks of one line batch share
+// the cache lines their 8*W-byte row segments straddle (K is odd), and they
+// now share one L2.
+__device__ __forceinline__ int xcd_logical(int id, int total)
+{
+    const int xcd = id % kXcds, full = total / kXcds, rem = total % kXcds;
+    return xcd * full + min(xcd, rem) + id / kXcds;
+}
+
+// Uniform-base tiles (fft_tile_uniform): one 1-D grid of main-field tiles
+// then extra-field tiles.
 template <int L, int R1, int R2, int W, int KIND>
 __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArgs a)
 {
     __shared__ double tile[L * W];
-    // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
-    // so XCD x is given the contiguous logical range [x*T/8, (x+1)*T/8) of
-    // tiles (lane block fastest).  Neighbouring lane blocks of one line batch
-    // share the cache lines their 8*W-byte row segments straddle (K is odd),
-    // and they now share one L2.
-    const int nbx = gridDim.x, total = nbx * gridDim.y;
-    const int id = blockIdx.x + nbx * blockIdx.y;
-    const int xcd = id % kXcds, full = total / kXcds, rem = total % kXcds;
-    const int logical = xcd * full + min(xcd, rem) + id / kXcds;
-    const int batch = logical / nbx;
-    fft_tile<L, R1, R2, W, KIND>(a, logical % nbx, batch, tile);
+    fft_tile_uniform<L, R1, R2, W, KIND>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
+}
+
+// Per-thread addressing, for launches whose offsets do not fit 32 bits.
+template <int L, int R1, int R2, int W, int KIND>
+__global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines_wide(const FftArgs a)
+{
+    __shared__ double tile[L * W];
+    const int nbx = gridDim.x, logical = xcd_logical(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
+    fft_tile<L, R1, R2, W, KIND>(a, logical % nbx, logical / nbx, tile);
+}
+
+// Largest element index a uniform-base tile of this launch forms, on either
+// side (inputs and outputs); below 2^29 the byte offsets fit 32 bits.
+bool offsets_fit_32(int L, int W, const FftArgs &a)
+{
+    const long long lim = 1LL << 29;
+    const long long m_in = (W - 1) + (long long)(L - 1) * a.in_point_stride;
+    const long long m_out = (W - 1) + (long long)(L - 1) * a.out_point_stride;
+    long long x_in = 0, x_out = 0;
+    if (a.xlanes > 0) {
+        x_in = (long long)(W - 1) * a.x_in_batch_stride + (long long)(L - 1) * a.x_in_point_stride;
+        x_out = (long long)(W - 1) * a.x_out_batch_stride + (long long)(L - 1) * a.x_out_point_stride;
+    }
+    return m_in < lim && m_out < lim && x_in < lim && x_out < lim && m_in >= 0 && m_out >= 0;
 }
 
 template <int L, int R1, int R2, int W>
 int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
 {
     constexpr int Q = R1 > R2 ? R1 : R2;
-    dim3 grid((a.lanes + a.xlanes + W - 1) / W, a.nbatch), block(W * Q);
-    switch (kind) {
-    case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
-    case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
-    case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a); break;
-    case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a); break;
+    const dim3 block(W * Q);
+    if (offsets_fit_32(L, W, a)) {
+        const long long tiles = (long long)a.nbatch * ((a.lanes + W - 1) / W) +
+                                (long long)a.xlanes * ((a.nbatch + W - 1) / W);
+        const dim3 grid((unsigned)tiles);
+        switch (kind) {
+        case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
+        case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
+        case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a); break;
+        case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a); break;
+        }
+    } else {
+        const dim3 grid((a.lanes + a.xlanes + W - 1) / W, a.nbatch);
+        switch (kind) {
+        case FFT_R2C: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
+        case FFT_C2R: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
+        case FFT_C2C_FWD:
+            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a);
+            break;
+        case FFT_C2C_INV:
+            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a);
+            break;
+        }
     }
     HZ_HIP(hipGetLastError());
     return HZ_OK;

@@ -15,6 +15,32 @@ constexpr int bitrev(int i, int bits)
 }
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Global-memory views of the field pointers (every FFT operand is device
+// memory; the explicit address space keeps global loads when a pointer is
+// chosen per thread).
+typedef const double __attribute__((address_space(1))) gcdouble;
+typedef double __attribute__((address_space(1))) gdouble;
+
+// A wave-uniform argument value (kernel argument, or a field of a problem
+// record read at a uniform index) moved to scalar registers: this keeps the
+// compiler from turning a per-thread choice between two arguments into a
+// per-thread load from the argument block, which every data load would then
+// wait behind.
+template <class T>
+__device__ __forceinline__ T sval(T v)
+{
+    static_assert(sizeof(T) == 8, "64-bit values");
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (T)(((unsigned long long)hi << 32) | lo);
+}
+
 // In-register R-point DFT, radix-2 decimation in time.  tw holds the
 // L-point table; W_R^t = tw[t * (L/R)].  SIGN = -1 forward, +1 inverse.
 template <int R, int L, int SIGN>
@@ -53,37 +79,48 @@ __device__ __forceinline__ void dft_regs(double (&re)[R], double (&im)[R], const
     }
 }
 
-// One (lane block, line batch) tile of the line transform; W * max(R1, R2)
-// threads; `tile` holds L * W doubles of LDS (used twice: real parts, then
-// imaginary parts).  Starts with a barrier so that back-to-back tiles can
-// reuse the LDS.
-template <int L, int R1, int R2, int W, int KIND>
-__device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int batch, double *tile)
+// Addressing of a tile's operands.  Point p of the thread's line is at
+// element index off(p) = base_offset + p * point_stride of each operand.
+// Addr64: per-thread pointers and 64-bit indices (any lane mix, any size).
+struct Addr64 {
+    gcdouble *ire, *iim;
+    gdouble *ore, *oim;
+    long long ib, ips, ob, ops;
+    __device__ __forceinline__ double ldr(int p) const { return ire[ib + (long long)p * ips]; }
+    __device__ __forceinline__ double ldi(int p) const { return iim[ib + (long long)p * ips]; }
+    __device__ __forceinline__ void str(int p, double v) const { ore[ob + (long long)p * ops] = v; }
+    __device__ __forceinline__ void sti(int p, double v) const { oim[ob + (long long)p * ops] = v; }
+};
+// Addr32: wave-uniform base pointers and 32-bit per-thread byte offsets (the
+// host guarantees every offset of the launch is below 2^32 bytes), so each
+// access is a scalar base plus one offset register.
+struct Addr32 {
+    gcdouble *ire, *iim;
+    gdouble *ore, *oim;
+    unsigned ib, ips, ob, ops;
+    template <class P>
+    __device__ __forceinline__ static P at(P base, unsigned idx)
+    {
+        typedef __attribute__((address_space(1))) char gchar;
+        return (P)((gchar *)base + 8u * idx);
+    }
+    __device__ __forceinline__ double ldr(int p) const { return *at(ire, ib + (unsigned)p * ips); }
+    __device__ __forceinline__ double ldi(int p) const { return *at(iim, ib + (unsigned)p * ips); }
+    __device__ __forceinline__ void str(int p, double v) const { *at(ore, ob + (unsigned)p * ops) = v; }
+    __device__ __forceinline__ void sti(int p, double v) const { *at(oim, ob + (unsigned)p * ops) = v; }
+};
+
+// The transform of one tile: W lines (thread lane w), L points each, W *
+// max(R1, R2) threads; `tile` holds L * W doubles of LDS (used twice: real
+// parts, then imaginary parts).  Starts with a barrier so that back-to-back
+// tiles can reuse the LDS.
+template <int L, int R1, int R2, int W, int KIND, class A>
+__device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *__restrict__ tw, double *tile)
 {
     static_assert(R1 * R2 == L, "radix split");
     constexpr int SIGN = (KIND == FFT_R2C || KIND == FFT_C2C_FWD) ? -1 : 1;
     __syncthreads();
     const int w = threadIdx.x % W, q = threadIdx.x / W;
-    const int lane = lane_block * W + w;
-    const double2 *__restrict__ tw = a.tw;
-    // lane source: the main field, or one of the extra single-lane fields
-    const double *in_re = a.in_re, *in_im = a.in_im;
-    double *out_re = a.out_re, *out_im = a.out_im;
-    long long ibase = (long long)batch * a.in_batch_stride + lane, ips = a.in_point_stride;
-    long long obase = (long long)batch * a.out_batch_stride + lane, ops = a.out_point_stride;
-    bool live = lane < a.lanes;
-    if (!live && lane < a.lanes + a.xlanes) {
-        const int sx = lane - a.lanes;
-        in_re = a.xin_re[sx];
-        in_im = a.xin_im[sx];
-        out_re = a.xout_re[sx];
-        out_im = a.xout_im[sx];
-        ibase = (long long)batch * a.x_in_batch_stride;
-        ips = a.x_in_point_stride;
-        obase = (long long)batch * a.x_out_batch_stride;
-        ops = a.x_out_point_stride;
-        live = true;
-    }
 
     double sre[R1], sim[R1];
     if (q < R2) {
@@ -94,38 +131,53 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
             double vr = 0.0, vi = 0.0;
             if (live) {
                 if (KIND == FFT_R2C) {
-                    vr = in_re[ibase + (long long)p * ips];
+                    vr = ad.ldr(p);
                 } else if (KIND == FFT_C2R) {
                     const int pp = p <= L / 2 ? p : L - p;
-                    const long long off = ibase + (long long)pp * ips;
-                    vr = in_re[off];
-                    vi = in_im[off];
+                    vr = ad.ldr(pp);
+                    vi = ad.ldi(pp);
                     if (p > L / 2) vi = -vi;
                 } else {
-                    const long long off = ibase + (long long)p * ips;
-                    vr = in_re[off];
-                    vi = in_im[off];
+                    vr = ad.ldr(p);
+                    vi = ad.ldi(p);
                 }
             }
             sre[n1] = vr;
             sim[n1] = vi;
         }
+        // inter-stage twiddles W_L^(n2 k1) = prod over the base-4 digits d_m
+        // of k1 of (W_L^(n2 4^m))^(d_m): one table load per digit position
+        // (loaded before any LDS store, which would otherwise serialise them)
+        // and a few complex products, instead of R1 - 1 loads held in
+        // registers.  At most four roundings per twiddle.
+        constexpr int ND = (ilog2(R1) + 1) / 2;
+        double2 pw[ND][4];
+#pragma unroll
+        for (int m = 0; m < ND; ++m) {
+            const double2 b = tw[(n2 << (2 * m)) % L];
+            pw[m][1] = b;
+            pw[m][2] = cmul(b, b);
+            pw[m][3] = cmul(pw[m][2], b);
+        }
         dft_regs<R1, L, SIGN>(sre, sim, tw);
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) {
-            double vr = sre[k1], vi = sim[k1];
-            const int t = (n2 * k1) % L;
-            if (t != 0) {
-                const double2 tt = tw[t];
-                const double wr = tt.x, wi = SIGN * tt.y;
-                const double xr = vr * wr - vi * wi;
-                vi = vr * wi + vi * wr;
-                vr = xr;
+        for (int k1 = 1; k1 < R1; ++k1) {
+            double2 t = make_double2(1.0, 0.0);
+            bool first = true;
+#pragma unroll
+            for (int m = 0; m < ND; ++m) {
+                const int d = (k1 >> (2 * m)) & 3;
+                if (d == 0) continue;
+                t = first ? pw[m][d] : cmul(t, pw[m][d]);
+                first = false;
             }
-            sre[k1] = vr;
-            sim[k1] = vi;
-            tile[(k1 * R2 + n2) * W + w] = vr;
+            const double vr = sre[k1], vi = sim[k1];
+            const double wr = t.x, wi = SIGN * t.y;
+            sre[k1] = vr * wr - vi * wi;
+            sim[k1] = vr * wi + vi * wr;
         }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) tile[(k1 * R2 + n2) * W + w] = sre[k1];
     }
     __syncthreads();
     double re[R2], im[R2];
@@ -145,29 +197,96 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
         for (int n2 = 0; n2 < R2; ++n2) im[n2] = tile[(k1 * R2 + n2) * W + w];
         dft_regs<R2, L, SIGN>(re, im, tw);
         if (live) {
-            const long long base = obase;
             constexpr double scale = 1.0 / L;
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) {
                 const int kk = k1 + R1 * k2;
-                const long long off = base + (long long)kk * ops;
                 if (KIND == FFT_R2C) {
                     if (kk <= L / 2) {
-                        out_re[off] = re[k2];
-                        out_im[off] = im[k2];
+                        ad.str(kk, re[k2]);
+                        ad.sti(kk, im[k2]);
                     }
                 } else if (KIND == FFT_C2R) {
-                    out_re[off] = re[k2] * scale;
+                    ad.str(kk, re[k2] * scale);
                 } else if (KIND == FFT_C2C_INV) {
-                    out_re[off] = re[k2] * scale;
-                    out_im[off] = im[k2] * scale;
+                    ad.str(kk, re[k2] * scale);
+                    ad.sti(kk, im[k2] * scale);
                 } else {
-                    out_re[off] = re[k2];
-                    out_im[off] = im[k2];
+                    ad.str(kk, re[k2]);
+                    ad.sti(kk, im[k2]);
                 }
             }
         }
     }
+}
+
+// One (lane block, line batch) tile, lanes = the main field's z lanes then
+// the extra single-lane fields (faces), per-thread addressing (the batch
+// kernels' tile).
+template <int L, int R1, int R2, int W, int KIND>
+__device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int batch, double *tile)
+{
+    const int w = threadIdx.x % W;
+    const int lane = lane_block * W + w;
+    // Chosen by selects between wave-uniform argument values (never by
+    // indexing them with a per-thread value, which turns into vector loads
+    // of the arguments that every data load then waits behind).
+    const bool xl = lane >= a.lanes && lane < a.lanes + a.xlanes;
+    const bool x1 = lane - a.lanes == 1;
+    Addr64 ad;
+    ad.ire = (gcdouble *)(xl ? (x1 ? sval(a.xin_re[1]) : sval(a.xin_re[0])) : sval(a.in_re));
+    ad.iim = (gcdouble *)(xl ? (x1 ? sval(a.xin_im[1]) : sval(a.xin_im[0])) : sval(a.in_im));
+    ad.ore = (gdouble *)(xl ? (x1 ? sval(a.xout_re[1]) : sval(a.xout_re[0])) : sval(a.out_re));
+    ad.oim = (gdouble *)(xl ? (x1 ? sval(a.xout_im[1]) : sval(a.xout_im[0])) : sval(a.out_im));
+    ad.ib = xl ? (long long)batch * sval(a.x_in_batch_stride) : (long long)batch * sval(a.in_batch_stride) + lane;
+    ad.ips = xl ? sval(a.x_in_point_stride) : sval(a.in_point_stride);
+    ad.ob = xl ? (long long)batch * sval(a.x_out_batch_stride) : (long long)batch * sval(a.out_batch_stride) + lane;
+    ad.ops = xl ? sval(a.x_out_point_stride) : sval(a.out_point_stride);
+    fft_core<L, R1, R2, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
+}
+
+// Uniform-base tiles of one launch (fft.hip): tile t < nbatch * nlb is lane
+// block t % nlb of line batch t / nlb of the main field; the rest are the
+// extra fields, xlanes * ceil(nbatch / W) tiles whose W "lanes" are W
+// consecutive line batches of one extra field.
+template <int L, int R1, int R2, int W, int KIND>
+__device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double *tile)
+{
+    const int w = threadIdx.x % W;
+    const int nlb = (a.lanes + W - 1) / W;
+    const int nmain = a.nbatch * nlb;
+    Addr32 ad;
+    bool live;
+    if (t < nmain) {
+        const int batch = t / nlb, l0 = (t % nlb) * W;
+        const long long ib = (long long)batch * a.in_batch_stride + l0;
+        const long long ob = (long long)batch * a.out_batch_stride + l0;
+        ad.ire = (gcdouble *)sval(a.in_re) + ib;
+        ad.iim = (gcdouble *)sval(a.in_im) + ib;
+        ad.ore = (gdouble *)sval(a.out_re) + ob;
+        ad.oim = (gdouble *)sval(a.out_im) + ob;
+        ad.ib = w;
+        ad.ob = w;
+        ad.ips = (unsigned)a.in_point_stride;
+        ad.ops = (unsigned)a.out_point_stride;
+        live = l0 + w < a.lanes;
+    } else {
+        const int nbb = (a.nbatch + W - 1) / W;
+        const int u = t - nmain, sx = u / nbb, b0 = (u % nbb) * W;
+        const bool x1 = sx == 1;
+        const long long ib = (long long)b0 * a.x_in_batch_stride;
+        const long long ob = (long long)b0 * a.x_out_batch_stride;
+        ad.ire = (gcdouble *)(x1 ? sval(a.xin_re[1]) : sval(a.xin_re[0])) + ib;
+        ad.iim = (gcdouble *)(x1 ? sval(a.xin_im[1]) : sval(a.xin_im[0])) + ib;
+        ad.ore = (gdouble *)(x1 ? sval(a.xout_re[1]) : sval(a.xout_re[0])) + ob;
+        ad.oim = (gdouble *)(x1 ? sval(a.xout_im[1]) : sval(a.xout_im[0])) + ob;
+        ad.ib = (unsigned)(w * a.x_in_batch_stride);
+        ad.ob = (unsigned)(w * a.x_out_batch_stride);
+        ad.ips = (unsigned)a.x_in_point_stride;
+        ad.ops = (unsigned)a.x_out_point_stride;
+        live = b0 + w < a.nbatch;
+    }
+    fft_core<L, R1, R2, W, KIND>(ad, live, a.tw, tile);
 }
 
 }  // namespace
