@@ -177,7 +177,7 @@ template <int L, int R1, int R2, int W, int KIND>
 __global__ __launch_bounds__(256) void batch_fft_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list,
                                                       int pass)
 {
-    __shared__ double tile[L * W];
+    __shared__ double tile[(L + 2) * W];
     const int2 t = list[blockIdx.x];
     const FftArgs &a = P[t.x].f[pass];
     const int lanes = a.lanes + a.xlanes;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void batch_diag_list(const BatchProblem *__res
 template <int RT>
 constexpr size_t batch_lds()
 {
-    constexpr size_t fft = sizeof(double) * 256 * 16;  // largest transform tile (L = 256, W = 16)
+    constexpr size_t fft = sizeof(double) * (256 + 2) * 16;  // largest transform tile (L = 256, W = 16)
     constexpr size_t dg = sizeof(DiagShared<8, RT, true, false>);
     return fft > dg ? fft : dg;
 }

@@ -159,12 +159,29 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     }
     __syncthreads();
     // ---- stage F^T: tile[row][k] = f interior of the row's element ----
-    for (int idx = tid; idx < RT * NP; idx += 256) {
-        const int row = idx / NP, k = idx % NP;
-        const long long col = row_col[row];
-        double v = 0.0;
-        if (col >= 0 && k < n) v = a.spec[col + 1 + k];
-        tile[row * LD + k] = v;
+    // Unrolled in chunks of up to 16 independent loads, all issued before
+    // any LDS store (a loop with one load per trip waits out the memory
+    // latency on every trip).
+    {
+        constexpr int NIT = RT * NP / 256;  // loads per thread
+        constexpr int CH = NIT % 16 == 0 ? 16 : NIT % 12 == 0 ? 12 : NIT % 8 == 0 ? 8 : NIT;
+        static_assert(NIT % CH == 0 && RT * NP % 256 == 0, "whole chunks");
+#pragma unroll
+        for (int c0 = 0; c0 < NIT; c0 += CH) {
+            double v[CH];
+#pragma unroll
+            for (int it = 0; it < CH; ++it) {
+                const int idx = tid + 256 * (c0 + it);
+                const int row = idx / NP, k = idx % NP;
+                const long long col = row_col[row];
+                v[it] = (col >= 0 && k < n) ? a.spec[col + 1 + k] : 0.0;
+            }
+#pragma unroll
+            for (int it = 0; it < CH; ++it) {
+                const int idx = tid + 256 * (c0 + it);
+                tile[(idx / NP) * LD + idx % NP] = v[it];
+            }
+        }
     }
     __syncthreads();
 

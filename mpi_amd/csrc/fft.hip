@@ -37,7 +37,7 @@ __device__ __forceinline__ int xcd_logical(int id, int total)
 template <int L, int R1, int R2, int W, int KIND>
 __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArgs a)
 {
-    __shared__ double tile[L * W];
+    __shared__ double tile[(L + 2) * W];
     fft_tile_uniform<L, R1, R2, W, KIND>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
 }
 
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArg
 template <int L, int R1, int R2, int W, int KIND>
 __global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines_wide(const FftArgs a)
 {
-    __shared__ double tile[L * W];
+    __shared__ double tile[(L + 2) * W];
     const int nbx = gridDim.x, logical = xcd_logical(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
     fft_tile<L, R1, R2, W, KIND>(a, logical % nbx, logical / nbx, tile);
 }

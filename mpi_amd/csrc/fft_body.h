@@ -111,8 +111,8 @@ struct Addr32 {
 };
 
 // The transform of one tile: W lines (thread lane w), L points each, W *
-// max(R1, R2) threads; `tile` holds L * W doubles of LDS (used twice: real
-// parts, then imaginary parts).  Starts with a barrier so that back-to-back
+// max(R1, R2) threads; `tile` holds (L + 2) * W doubles of LDS (the C2R
+// half spectrum; then the exchange, real parts, then imaginary parts).  Starts with a barrier so that back-to-back
 // tiles can reuse the LDS.
 template <int L, int R1, int R2, int W, int KIND, class A>
 __device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *__restrict__ tw, double *tile)
@@ -123,28 +123,58 @@ __device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *
     const int w = threadIdx.x % W, q = threadIdx.x / W;
 
     double sre[R1], sim[R1];
-    if (q < R2) {
-        const int n2 = q;
+    if constexpr (KIND == FFT_C2R) {
+        // The half spectrum (points 0..L/2) is read once into the tile
+        // (real parts, then imaginary parts), and the mirrored points
+        // L/2+1..L-1 are taken from there conjugated: each point is loaded
+        // from memory once instead of twice.
+        constexpr int H = L / 2 + 1, Q = R1 > R2 ? R1 : R2;
+        constexpr int NJ = (H + Q - 1) / Q;
+        double hr[NJ], hi[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int pt = q + Q * j;
+            hr[j] = hi[j] = 0.0;
+            if (live && pt < H) {
+                hr[j] = ad.ldr(pt);
+                hi[j] = ad.ldi(pt);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int pt = q + Q * j;
+            if (pt < H) {
+                tile[pt * W + w] = hr[j];
+                tile[(H + pt) * W + w] = hi[j];
+            }
+        }
+        __syncthreads();
+        if (q < R2) {
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) {
+                const int p = q + R2 * n1;
+                const int pp = p <= L / 2 ? p : L - p;
+                sre[n1] = tile[pp * W + w];
+                const double vi = tile[(H + pp) * W + w];
+                sim[n1] = p > L / 2 ? -vi : vi;
+            }
+        }
+        __syncthreads();  // every staged point is read before the tile is reused
+    } else if (q < R2) {
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-            const int p = n2 + R2 * n1;
+            const int p = q + R2 * n1;
             double vr = 0.0, vi = 0.0;
             if (live) {
-                if (KIND == FFT_R2C) {
-                    vr = ad.ldr(p);
-                } else if (KIND == FFT_C2R) {
-                    const int pp = p <= L / 2 ? p : L - p;
-                    vr = ad.ldr(pp);
-                    vi = ad.ldi(pp);
-                    if (p > L / 2) vi = -vi;
-                } else {
-                    vr = ad.ldr(p);
-                    vi = ad.ldi(p);
-                }
+                vr = ad.ldr(p);
+                if (KIND != FFT_R2C) vi = ad.ldi(p);
             }
             sre[n1] = vr;
             sim[n1] = vi;
         }
+    }
+    if (q < R2) {
+        const int n2 = q;
         // inter-stage twiddles W_L^(n2 k1) = prod over the base-4 digits d_m
         // of k1 of (W_L^(n2 4^m))^(d_m): one table load per digit position
         // (loaded before any LDS store, which would otherwise serialise them)

@@ -71,6 +71,40 @@ def counter_by_kernel(path, counter):
     return per
 
 
+def counters_summary(paths, prefix):
+    """Median per-dispatch value of every counter in the given rocprofv3
+    databases, for kernels whose short name starts with `prefix`, plus
+    derived ratios when their inputs are present (MFMA busy share of the
+    SIMD cycles, wait share of the wave cycles, LDS bank conflicts per LDS
+    access cycle)."""
+    out = {}
+    for path in paths:
+        db = sqlite3.connect(path)
+        q = ("select k.name, p.counter_name, p.counter_value from pmc_events p join kernels k "
+             "on k.dispatch_id = p.dispatch_id")
+        vals = {}
+        for name, cname, v in db.execute(q):
+            sn = short_name(name)
+            if sn.startswith(prefix):
+                vals.setdefault(sn, {}).setdefault(cname, []).append(float(v))
+        for sn, cs in vals.items():
+            d = out.setdefault(sn, {})
+            for cname, vs in cs.items():
+                d[cname] = statistics.median(vs)
+                d.setdefault("dispatches", len(vs))
+    for d in out.values():
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in d and "SQ_BUSY_CU_CYCLES" in d and d["SQ_BUSY_CU_CYCLES"]:
+            # busy-CU cycles are per CU; four SIMDs each carry one MFMA pipe
+            d["mfma_busy_per_simd_cycle"] = d["SQ_VALU_MFMA_BUSY_CYCLES"] / (4 * d["SQ_BUSY_CU_CYCLES"])
+        if "SQ_WAIT_ANY" in d and "SQ_WAVE_CYCLES" in d and d["SQ_WAVE_CYCLES"]:
+            d["wait_any_per_wave_cycle"] = d["SQ_WAIT_ANY"] / d["SQ_WAVE_CYCLES"]
+        if "SQ_WAIT_INST_LDS" in d and "SQ_WAVE_CYCLES" in d and d["SQ_WAVE_CYCLES"]:
+            d["wait_lds_per_wave_cycle"] = d["SQ_WAIT_INST_LDS"] / d["SQ_WAVE_CYCLES"]
+        if "SQ_LDS_BANK_CONFLICT" in d and "SQ_LDS_IDX_ACTIVE" in d and d["SQ_LDS_IDX_ACTIVE"]:
+            d["lds_bank_conflict_per_active"] = d["SQ_LDS_BANK_CONFLICT"] / d["SQ_LDS_IDX_ACTIVE"]
+    return out
+
+
 def _probe_value(per):
     for name, vals in per.items():
         if "MulFunctor" in name:
@@ -111,7 +145,13 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--copy-mib", type=int, required=True)
     t.add_argument("--out")
+    c = sub.add_parser("counters")
+    c.add_argument("db", nargs="+")
+    c.add_argument("--kernel", default="", help="short-name prefix of the kernels to report")
     a = ap.parse_args(argv)
+    if a.cmd == "counters":
+        print(json.dumps(counters_summary(a.db, a.kernel), indent=1, sort_keys=True))
+        return
     if a.cmd == "stats":
         rows = kernel_stats(a.csv)[: a.top]
         print(f"{'kernel':70s} {'calls':>6s} {'avg_us':>10s} {'total_ms':>10s} {'%':>6s}")
