@@ -1,0 +1,46 @@
+// NonPeriodic2D_MPI.h (drop-in) -- the interface of gpritam/MPI
+// HelmholtzSolvers/DirectPseudoSpectral/include/NonPeriodic2D_MPI.h:15-46
+// (periodic x, GLL elements in y, Robin y faces), same names, globals and
+// argument meaning, running on one GPU through the C ABI of hz.h (a plan with
+// nx = 1, ny = Nx).  One process holds every element (Rank = 0, size = 1), so
+// Ns = ceil(Ny/N) elements of N GLL points and y runs from 0 as in the
+// reference (NonPeriodic2D_MPI.cpp:95-112).
+//
+// Differences a caller can observe: SolvePDE leaves rhsPDE, leftBC and
+// rightBC unchanged; `determinant` is not updated while SolvePDE records
+// operators; SolverForm selects the GPU method (HZ_FORM_PLAIN_LU by default,
+// HZ_FORM_DIAG for the standard shifted operator).
+#pragma once
+
+#include "../hz.h"
+#include "General.h"
+#include "LU_Decomposition.h"
+#include "Fourier_Quadrature.h"
+
+struct FieldVariable {
+    double **phi;
+};
+
+extern int Rank, size;
+extern int Nx, Ny, NX, N, Ns;
+extern double xl, xr, yl, yr, Lx, Ly, dx;
+extern double alphal, betal, alphar, betar, Lambda;
+extern double **Dy, **Dyy, **Operator, *y, ***rhsPDE, *leftBC, *rightBC, determinant;
+extern int *npivot;
+extern int SolverForm;
+
+void CreatePDEPlan(const int Nx, const int Ny, const int N, const double xl, const double xr, const double yl,
+                   const double yr);
+void DestroyPDEPlan();
+
+void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int));
+
+// The reference leaves the X versions empty (NonPeriodic2D_MPI.cpp:479-492);
+// here they are Fourier derivatives along x on the GPU.
+void FirstDerivativeX(FieldVariable Ux[], FieldVariable U[]);
+void SecondDerivativeX(FieldVariable Uxx[], FieldVariable U[]);
+void FirstDerivativeY(FieldVariable Uy[], FieldVariable U[]);
+void SecondDerivativeY(FieldVariable Uyy[], FieldVariable U[]);
+
+void ComputeError(FieldVariable U[], double (*ExactFunction)(const double, const double));
+void WriteFile(FieldVariable U[], double (*ExactFunction)(const double, const double));

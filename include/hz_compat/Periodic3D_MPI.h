@@ -1,0 +1,53 @@
+// Periodic3D_MPI.h (drop-in) -- the interface of gpritam/MPI
+// HelmholtzSolvers/DirectPseudoSpectral/include/Periodic3D_MPI.h:15-46 (z
+// periodic), same names, globals and argument meaning, running on one GPU
+// through the C ABI of hz.h (a plan with periodic_z).  One process holds
+// every element (Rank = 0, size = 1), so Ns = ceil(Nz/N) elements of N GLL
+// points and z runs from 0 as in the reference (Periodic3D_MPI.cpp:95-121).
+//
+// Differences a caller can observe:
+//  - SolvePDE leaves rhsPDE unchanged (the reference overwrites it with its
+//    spectrum);
+//  - alphaPDE (the reference's per-column closure coefficients of the cyclic
+//    interface system) is allocated but not filled: the closure is formed
+//    on the GPU inside the solve;
+//  - `determinant` is not updated while SolvePDE records operators;
+//  - SolverForm selects the GPU method (HZ_FORM_PLAIN_LU, the reference
+//    algorithm, by default; HZ_FORM_DIAG for the z-diagonalised form when the
+//    callback builds the standard shifted operator).
+#pragma once
+
+#include "../hz.h"
+#include "General.h"
+#include "LU_Decomposition.h"
+#include "Fourier_Quadrature.h"
+
+struct FieldVariable {
+    double ***phi;
+};
+
+extern int Rank, size;
+extern int Nx, Ny, Nz, NY, N, Ns;
+extern double xl, xr, yl, yr, zl, zr, Lx, Ly, Lz, dx, dy, Lambda;
+extern double **Dz, **Dzz, **Operator, *z, ****rhsPDE, determinant, **alphaPDE;
+extern int *npivot;
+extern int SolverForm;
+
+void CreatePDEPlan(const int Nx, const int Ny, const int Nz, const int N, const double xl, const double xr,
+                   const double yl, const double yr, const double zl, const double zr);
+void DestroyPDEPlan();
+
+void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int, const int));
+
+// The reference leaves the X/Y versions empty (Periodic3D_MPI.cpp:869-900);
+// here they are the Fourier derivatives of Fourier_Quadrature.cpp:1494-1800
+// applied to every element, on the GPU.
+void FirstDerivativeX(FieldVariable Ux[], FieldVariable U[]);
+void SecondDerivativeX(FieldVariable Uxx[], FieldVariable U[]);
+void FirstDerivativeY(FieldVariable Uy[], FieldVariable U[]);
+void SecondDerivativeY(FieldVariable Uyy[], FieldVariable U[]);
+void FirstDerivativeZ(FieldVariable Uz[], FieldVariable U[]);
+void SecondDerivativeZ(FieldVariable Uzz[], FieldVariable U[]);
+
+void ComputeError(FieldVariable U[], double (*ExactFunction)(const double, const double, const double));
+void WriteFile(FieldVariable U[], double (*ExactFunction)(const double, const double, const double));

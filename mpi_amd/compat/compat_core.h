@@ -1,0 +1,60 @@
+// compat_core.h -- the engine shared by the drop-in solvers
+// (NonPeriodic3D.cpp, Periodic3D.cpp, NonPeriodic2D.cpp, NonPeriodic1D.cpp):
+// error handling, the operator recorder behind LUDecomposition, and one GPU
+// plan with pinned staging in the plan's layout.  Internal to the drop-in
+// libraries; the reference-named functions (LUDecomposition, LUSolve,
+// ErrorMessage, CreateFFTPlan, DestroyFFTPlan) are defined in
+// compat_core.cpp.
+#pragma once
+
+#include "../../include/hz.h"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace hzc {
+
+void hip_check(hipError_t e, const char *what);
+void hz_check(int code);  // ErrorMessage(hz_last_error_string()) on failure
+
+// While a SolvePDE runs the caller's ConstructOperator callbacks,
+// LUDecomposition on the solver's Operator is recorded here instead of
+// factored: the diagonal alone while every off-diagonal entry equals the
+// interior of the second-derivative matrix (the usual shifted operator), the
+// whole matrix from the first callback that differs.
+struct Recorder {
+    bool on = false, full = false;
+    int mode = -1, modes = 0, n = 0;
+    double **op = nullptr;   // the solver's Operator
+    double **dxx = nullptr;  // its N x N second-derivative matrix
+    std::vector<double> diag, ops;
+    void begin(int modes_, int n_, double **op_, double **dxx_);
+    bool take(double **A, int n_);  // false: not a matrix being recorded
+};
+Recorder &recorder();
+
+// Size contract of the reference's FFT plan (Fourier_Quadrature.cpp:16-69):
+// -1 when none was created.
+int fft_plan_size();
+
+// One GPU plan, with pinned host staging in its layout: h_f holds f (and
+// then u) as [nx][ny][K]; h_g the faces [2][nx][ny].
+struct Solver {
+    hz_plan *plan = nullptr;
+    int K = 0, modes = 0, nx = 0, ny = 0, N = 0, ne = 0;
+    bool periodic = false;
+    double *d_f = nullptr, *d_gl = nullptr, *d_gr = nullptr, *d_u = nullptr, *d_ops = nullptr;
+    double *h_f = nullptr, *h_g = nullptr;
+    void create(const hz_plan_desc &d);
+    void destroy();
+    // Solve with the recorded operators: h_f / h_g in, u back into h_f.
+    void solve_recorded(double lambda, double al, double bl, double ar, double br);
+    // Element-array derivative along z: host [ne][nx][ny][N], in place.
+    void derivative_z(std::vector<double> &elements, int order);
+    // Fourier derivative along axis 0 (x) or 1 (y) of a host stack
+    // [nx][ny][lanes], in place.
+    void derivative_xy(std::vector<double> &stack, int lanes, int axis, int order);
+};
+
+}  // namespace hzc

@@ -46,3 +46,41 @@ def test_singular_operator_exits_like_reference(cuda):
     r = run("singular")
     assert r.returncode == 1
     assert "ERROR: Matrix is singular!" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# The other reference solvers' drop-ins (include/hz_compat/Periodic3D_MPI.h,
+# NonPeriodic2D_MPI.h, Periodic2D_MPI.h, NonPeriodic1D_MPI.h,
+# Periodic1D_MPI.h), each driven by a caller written like the reference's own
+# test program for that solver.  The printed error against the exact
+# solution must equal the reference's on the same problem (golden vectors,
+# tests/golden/make_golden.py) up to the difference of the two solutions,
+# which the GPU parity tests bound by max(1e-12, 2 x rounding spread).
+
+COMPAT = os.path.join(os.path.dirname(__file__), "..", "mpi_amd", "compat")
+
+
+def _bar(rec):
+    return max(1e-12, 2 * rec.get("rounding_spread", 0.0)) * rec["max_abs_u"]
+
+
+@pytest.mark.parametrize("exe,fixtures", [
+    ("TestPeriodic3D", ["psolve_shipped_np1.json"]),
+    ("TestNonPeriodic2D", ["low_2d_shipped_np1.json"]),
+    ("TestPeriodic2D", ["low_p2d_shipped_np1.json"]),
+    ("TestNonPeriodic1D", ["low_1d_shipped_a_np1.json", "low_1d_shipped_b_np1.json"]),
+    ("TestPeriodic1D", ["low_p1d_shipped_np1.json"]),
+])
+def test_dropin_reference_programs(cuda, exe, fixtures):
+    from golden_util import load
+    r = subprocess.run([os.path.join(COMPAT, exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    errs = [float(v) for v in re.findall(r"^Error = (\S+)", r.stdout, re.M)]
+    assert len(errs) == len(fixtures), r.stdout
+    for err, name in zip(errs, fixtures):
+        rec = load(name)
+        print(f"{exe}: Error = {err:.8e}, reference {rec['err_vs_exact']:.8e}")
+        assert abs(err - rec["err_vs_exact"]) <= _bar(rec) + 1e-7 * rec["err_vs_exact"]
+    m = re.search(r"SecondDerivativeX error = (\S+)", r.stdout)
+    if m:  # 2-D: d2/dx2 of the solution (Fourier along x) vs -(8 pi)^2 u
+        assert float(m.group(1)) <= 1e-6, m.group(1)
