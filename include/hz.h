@@ -208,6 +208,12 @@ int hz_batch_tuning(const hz_batch *batch, int *variant, int *fused, int *tile_r
  * studies. */
 int hz_batch_force_variant(hz_batch *batch, int variant);
 int hz_batch_destroy(hz_batch *batch);
+/* One-shot form of the above: create the batch, solve, wait for it,
+ * destroy it.  Returns when u[] is complete.
+ * Every call autotunes and records its graph again, so a caller that solves
+ * the same set repeatedly keeps an hz_batch instead. */
+int hz_solve_batched(hz_plan *const *plans, int count, const double *const *f, const double *const *gl,
+                     const double *const *gr, double *const *u, hz_stream_t stream);
 
 /* Refinement history of the last MIXED solve (synchronises the device):
  * rel_residual[s] = max over modes and columns of |b - A x|_max / |b|_max

@@ -61,6 +61,8 @@ SIGNATURES = {
     "hz_batch_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                                       ctypes.POINTER(_VP), _VP]),
     "hz_batch_destroy": (ctypes.c_int, [_VP]),
+    "hz_solve_batched": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
+                                        ctypes.POINTER(_VP), ctypes.POINTER(_VP), _VP]),
     "hz_batch_status": (ctypes.c_int, [_VP]),
     "hz_batch_tuning": (ctypes.c_int, [_VP, _PI, _PI, _PI, _PI, _PD, ctypes.c_int]),
     "hz_batch_force_variant": (ctypes.c_int, [_VP, ctypes.c_int]),

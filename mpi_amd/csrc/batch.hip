@@ -418,6 +418,8 @@ int upload_problems(hz_batch *b, const double *const *f, const double *const *gl
 void free_batch(hz_batch *b)
 {
     if (!b) return;
+    // work enqueued by hz_batch_solve may still read the tables below
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (hipEvent_t e : {b->ev_in, b->ev_out, b->t0, b->t1})
@@ -617,6 +619,19 @@ int hz_batch_destroy(hz_batch *b)
 {
     free_batch(b);
     return HZ_OK;
+}
+
+int hz_solve_batched(hz_plan *const *plans, int count, const double *const *f, const double *const *gl,
+                     const double *const *gr, double *const *u, hz_stream_t stream)
+{
+    hz_batch *b = nullptr;
+    int st = hz_batch_create(plans, count, &b);
+    if (st) return st;
+    st = hz_batch_solve(b, f, gl, gr, u, stream);
+    if (!st) st = hz_batch_status(b);
+    if (!st) HZ_HIP(hipStreamSynchronize((hipStream_t)stream));
+    free_batch(b);
+    return st;
 }
 
 }  // extern "C"

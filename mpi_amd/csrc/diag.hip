@@ -6,7 +6,7 @@ namespace hz {
 namespace {
 
 template <int NJ, int RT, bool GENERAL, bool ADJ>
-__global__ __launch_bounds__(256, 2) void diag_modes_kernel(const DiagArgs a)
+__global__ __launch_bounds__(256, RT >= 128 ? 1 : 2) void diag_modes_kernel(const DiagArgs a)
 {
     __shared__ DiagShared<NJ, RT, GENERAL, ADJ> sh;
     diag_tile<NJ, RT, GENERAL, ADJ>(a, blockIdx.x, sh);
@@ -46,7 +46,8 @@ int diag_padded(int n)
 int diag_tile_choices(int NP, int *rt, int max)
 {
     // 16 rows: more work-groups in flight to hide the staging latency;
-    // 128 rows (N-2 <= 64 only) holds a column of up to 64 elements
+    // 128 rows (N-2 <= 64 only) holds a column of up to 64 elements (82 KB
+    // of LDS: one work-group per CU)
     static const int c64[] = {16, 32, 64, 128}, c128[] = {16, 32, 48, 64}, c256[] = {16, 32};
     const int *c = NP == 64 ? c64 : NP == 128 ? c128 : c256;
     const int k = NP == 256 ? 2 : 4;

@@ -155,3 +155,26 @@ def test_batch_rejects_bad_sets(cuda):
     with pytest.raises(HzError):
         b.force_variant(0)  # fused variants are switched off
     b.close()
+
+
+def test_one_shot_batched_solve(cuda):
+    """hz_solve_batched (create, solve, wait, destroy in one call) equals the
+    single-plan solves, like a kept batch."""
+    import ctypes
+    from mpi_amd._lib import check, load
+    plans, inputs, _ = make_set(MIXED_SET)
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [dev(g) for _, g, _ in inputs]
+    grs = [dev(g) for _, _, g in inputs]
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    us = [p.empty_field() for p in plans]
+    n = len(plans)
+    VP = ctypes.c_void_p
+    arr = lambda xs: (VP * n)(*[VP(x) for x in xs])  # noqa: E731
+    check(load().hz_solve_batched(arr([p.handle.value for p in plans]), n, arr([t.data_ptr() for t in fs]),
+                                  arr([t.data_ptr() for t in gls]), arr([t.data_ptr() for t in grs]),
+                                  arr([t.data_ptr() for t in us]), None))
+    for k, (u, w) in enumerate(zip(us, want)):
+        assert rel(u.cpu().numpy(), w) <= 1e-13, k
+    for p in plans:
+        p.close()
