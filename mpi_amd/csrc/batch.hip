@@ -173,15 +173,20 @@ __global__ __launch_bounds__(256, 2) void batch_fused_kernel(const BatchProblem 
 // or mode-solve variant), each work-group one task from a (problem, task)
 // list; the launch boundaries order the stages, and every kernel inlines only
 // its own body, so it gets its own register budget.
+// One uniform-base tile per work-group (fft_tile_uniform: a lane block of a
+// line batch, or W line batches of a face), entries in XCD-aware order.
 template <int L, int R1, int R2, int W, int KIND>
 __global__ __launch_bounds__(256) void batch_fft_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list,
                                                       int pass)
 {
     __shared__ double tile[(L + 2) * W];
-    const int2 t = list[blockIdx.x];
+    const int2 t = list[xcd_logical(blockIdx.x, gridDim.x)];
     const FftArgs &a = P[t.x].f[pass];
-    const int lanes = a.lanes + a.xlanes;
-    for (int lb = 0; lb * W < lanes; ++lb) fft_tile<L, R1, R2, W, KIND>(a, lb, t.y, tile);
+    // The lists count both face tiles in the forward passes; a table filled
+    // without faces simply ends here instead of reading null face pointers.
+    const int tiles = a.nbatch * ((a.lanes + W - 1) / W) + a.xlanes * ((a.nbatch + W - 1) / W);
+    if (t.y >= tiles) return;
+    fft_tile_uniform<L, R1, R2, W, KIND>(a, t.y, tile);
 }
 
 template <int NJ, int RT, bool GENERAL>
@@ -308,6 +313,9 @@ int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, 
     return HZ_OK;
 }
 
+// z lanes per work-group of the batch's transform tiles of length L.
+int batch_lanes(int L) { return L <= 16 ? 64 : L <= 64 ? 32 : 16; }
+
 // Kind of a transform task: log2 of its length; of a mode-solve task: dkind.
 int task_kind(const BatchProblem &q, int s) { return s == 2 ? q.dkind : (s == 0 || s == 4 ? q.ly : q.lx); }
 
@@ -381,7 +389,20 @@ int build_lists(hz_batch *b, int rt)
     for (int p = 0; p < b->n; ++p)
         for (int st = 0; st < kStages; ++st) {
             auto &v = by[{st, task_kind(b->h_probs[p], st)}];
-            for (int t = 0; t < counts[p][st]; ++t) v.push_back(make_int2(p, t));
+            int ntask = counts[p][st];
+            if (st != 2) {  // transform passes: one entry per uniform-base tile
+                // The table is filled with null buffers here, so the face
+                // count is set from what hz_batch_solve always passes: both
+                // faces in the forward passes (its gl, gr are never null),
+                // none in the inverse ones.  Face offsets are below nx*M.
+                const FftArgs &a = b->h_probs[p].f[st < 2 ? st : st - 1];
+                const int L = task_kind(b->h_probs[p], st), W = batch_lanes(L), faces = st < 2 ? 2 : 0;
+                const long long lim = 1LL << 29;  // element offsets whose byte offsets fit 32 bits
+                const long long m = (W - 1) + (long long)(L - 1) * std::max(a.in_point_stride, a.out_point_stride);
+                if (m >= lim) return set_error(HZ_ERR_UNSUPPORTED, "batch: problem too large for a tile");
+                ntask = a.nbatch * ((a.lanes + W - 1) / W) + faces * ((a.nbatch + W - 1) / W);
+            }
+            for (int t = 0; t < ntask; ++t) v.push_back(make_int2(p, t));
         }
     std::vector<int2> all;
     b->groups[r].clear();

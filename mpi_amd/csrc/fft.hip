@@ -19,19 +19,6 @@
 namespace hz {
 namespace {
 
-constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
-
-// XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs, so
-// XCD x is given the contiguous logical range [x*T/8, (x+1)*T/8) of tiles
-// (lane block fastest).  Neighbouring lane blocks of one line batch share
-// the cache lines their 8*W-byte row segments straddle (K is odd), and they
-// now share one L2.
-__device__ __forceinline__ int xcd_logical(int id, int total)
-{
-    const int xcd = id % kXcds, full = total / kXcds, rem = total % kXcds;
-    return xcd * full + min(xcd, rem) + id / kXcds;
-}
-
 // Uniform-base tiles (fft_tile_uniform): one 1-D grid of main-field tiles
 // then extra-field tiles.
 template <int L, int R1, int R2, int W, int KIND>

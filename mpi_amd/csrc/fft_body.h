@@ -15,6 +15,19 @@ constexpr int bitrev(int i, int bits)
 }
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, one L2 each
+
+// XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs, so
+// XCD x is given the contiguous logical range [x*T/8, (x+1)*T/8) of tiles
+// (lane block fastest).  Neighbouring lane blocks of one line batch share
+// the cache lines their 8*W-byte row segments straddle (K is odd), and they
+// then share one L2.
+__device__ __forceinline__ int xcd_logical(int id, int total)
+{
+    const int xcd = id % kXcds, full = total / kXcds, rem = total % kXcds;
+    return xcd * full + min(xcd, rem) + id / kXcds;
+}
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
