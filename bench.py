@@ -64,13 +64,14 @@ def fft_bytes(nx, ny, K):
     return 2 * (real + half) + 2 * (2 * half)
 
 
-def whole_solve_roofline(form, ms_per_solve, nx, ny, n_gll, refine_sweeps=3):
+def whole_solve_roofline(form, ms_per_solve, nx, ny, n_gll, refine_sweeps=3, ceilings=None):
     """Whole-solve roofline: sum over the phases (forward transform, mode
     solve, inverse transform) of max(flop / FP64 peak, bytes / HBM peak),
     divided by the measured time per solve.  Bytes per transform are the
     three-kernel figure (read the field, write the half spectrum, or the
     reverse); `frac_executed` counts the two line passes per direction this
-    build runs instead."""
+    build runs instead, and `frac_executed_attainable` prices those against
+    the measured ceilings (FP64 MFMA rate, copy rate at the field size)."""
     K = n_gll
     real = nx * ny * K * 8
     half = nx * (ny // 2 + 1) * K * 16
@@ -90,11 +91,19 @@ def whole_solve_roofline(form, ms_per_solve, nx, ny, n_gll, refine_sweeps=3):
     executed = dict(ideal)
     # R2C (read real, write half) + C2C in place (read and write half); the reverse
     executed["fft_forward"] = executed["fft_inverse"] = (real + 3 * half) / PEAK_HBM * 1e3
-    return {"ideal_ms": sum(ideal.values()), "measured_ms": ms_per_solve,
-            "frac": sum(ideal.values()) / ms_per_solve,
-            "frac_executed": sum(executed.values()) / ms_per_solve,
-            "phase_ideal_ms": ideal,
-            "note": "sum_phase max(flop/78.6 TFLOP/s, bytes/8 TB/s) / measured ms per solve"}
+    out = {"ideal_ms": sum(ideal.values()), "measured_ms": ms_per_solve,
+           "frac": sum(ideal.values()) / ms_per_solve,
+           "frac_executed": sum(executed.values()) / ms_per_solve,
+           "phase_ideal_ms": ideal,
+           "note": "sum_phase max(flop/78.6 TFLOP/s, bytes/8 TB/s) / measured ms per solve"}
+    if ceilings:
+        peak_f = ceilings["fp64_mfma_tflops"] * 1e12
+        peak_b = ceilings["copy_gbps_field_size"] * 1e9
+        att = {k: max(fl / peak_f, by / peak_b) * 1e3 for k, (fl, by) in phases.items()}
+        att["fft_forward"] = att["fft_inverse"] = (real + 3 * half) / peak_b * 1e3
+        out["frac_executed_attainable"] = sum(att.values()) / ms_per_solve
+        out["phase_executed_attainable_ms"] = att
+    return out
 
 
 STAGE_KERNELS = {"diag_modes": "diag_modes_kernel", "lu_modes": "lu_modes_kernel",
@@ -114,6 +123,20 @@ def measured_traffic(stage):
     cands = [(v["dispatches"], v["hbm_bytes_per_dispatch"]) for k, v in kernels.items()
              if k.startswith(prefix) and "true>" not in k and v.get("hbm_bytes_per_dispatch")]
     return max(cands)[1] if cands else None
+
+
+def measure_ceilings(field_bytes):
+    """Attainable ceilings measured on this device (hz_probe_ceilings): the
+    FP64 MFMA rate of the mode solve's instruction with no memory traffic,
+    and device copy rates at the solve's field size and at 2 GiB."""
+    import ctypes
+    from mpi_amd import _lib
+    lib = _lib.load()
+    mfma, copy_field, copy_big = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+    _lib.check(lib.hz_probe_ceilings(ctypes.byref(mfma), ctypes.byref(copy_field), int(field_bytes)))
+    _lib.check(lib.hz_probe_ceilings(None, ctypes.byref(copy_big), 1 << 31))
+    return {"fp64_mfma_tflops": mfma.value, "copy_gbps_field_size": copy_field.value,
+            "copy_gbps_2GiB": copy_big.value, "field_bytes": int(field_bytes)}
 
 
 def make_inputs(plan):
@@ -421,6 +444,10 @@ def main():
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = measured_traffic(stage)
     roof["kernel_ms"] = ms
+    ceil = measure_ceilings(8 * NX * NY * K)
+    roof["attainable"] = ceil
+    roof["frac_of_attainable"] = roof["achieved"] / (
+        ceil["fp64_mfma_tflops"] if roof["bound"] == "mfma" else ceil["copy_gbps_field_size"])
 
     line = {
         "metric": "Helmholtz solves/s at 256^2x129, chosen vs plain form at rel err <=1e-10",
@@ -436,7 +463,8 @@ def main():
                   for k, v in results.items()},
         "chosen_vs_plain": (chosen["solves_per_s"] / results["plain"]["solves_per_s"]) if "plain" in results else None,
         "roofline": roof,
-        "whole_solve_roofline": whole_solve_roofline(chosen["form"], chosen["ms_per_solve"], NX, NY, N),
+        "whole_solve_roofline": whole_solve_roofline(chosen["form"], chosen["ms_per_solve"], NX, NY, N,
+                                                     ceilings=roof["attainable"]),
     }
     if not args.no_extras and "diag" in results:
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])

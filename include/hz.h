@@ -222,12 +222,19 @@ int hz_solve_batched(hz_plan *const *plans, int count, const double *const *f, c
  * No counterpart in the reference (its LU is FP64 only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
 /* Plan-time autotuning results: tile rows of the diagonalised mode kernel
- * (DIAG plans; 0 otherwise) with the measured ms of each candidate, and the
+ * (DIAG plans; 0 otherwise) with the measured ms of each candidate (0 for
+ * a candidate that does not apply), and the
  * z lanes per work-group chosen for each of the four transform passes
  * (R2C y, C2C x, C2C^-1 x, C2R y; 0 = the length's only shape) with the
- * winner's ms.  Any pointer may be NULL. */
+ * winner's ms.  Any pointer may be NULL.  No counterpart in the reference. */
 int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *diag_ms, int max_diag,
                    int *fft_lanes, double *fft_ms);
+/* The diagonalised mode kernel's candidate shapes, in the order of
+ * hz_plan_tuning's diag_ms (*count = their number, at most `max` written),
+ * and a way to pin one of them (it must hold one column's 2*ne element
+ * rows).  For tests and tuning studies; no counterpart in the reference. */
+int hz_plan_diag_candidates(const hz_plan *plan, int *tile_rows, int max, int *count);
+int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
 /* Number of modes of a MIXED plan that keep an FP64 operator. */
 int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 
@@ -249,6 +256,14 @@ int hz_host_eigen(int n, int ne, double lz, double *mu, double *V, double *Vinv)
 int hz_plan_profile(hz_plan *plan, int enable);
 int hz_plan_profile_read(hz_plan *plan, double *stage_ms, int max_stages, int *nstages,
                          int *nsolves, char *names, int names_len);
+
+/* Attainable ceilings of the current device, measured on the spot (about
+ * 10 ms): the FP64 matrix rate of v_mfma_f64_16x16x4f64 at full occupancy
+ * with no memory traffic (TFLOP/s), and the rate of a device-to-device copy
+ * of copy_bytes per buffer (GB/s, bytes read + written).  Either output
+ * may be NULL.  Allocates scratch buffers; synchronises.  A measurement
+ * hook with no counterpart in the reference. */
+int hz_probe_ceilings(double *fp64_mfma_tflops, double *copy_gbps, size_t copy_bytes);
 
 const char *hz_last_error_string(void);
 const char *hz_version(void);

@@ -69,12 +69,15 @@ SIGNATURES = {
     "hz_mixed_history": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI]),
     "hz_mixed_fp64_modes": (ctypes.c_int, [_VP, _PI]),
     "hz_plan_tuning": (ctypes.c_int, [_VP, _PI, _PD, ctypes.c_int, _PI, _PD]),
+    "hz_plan_diag_candidates": (ctypes.c_int, [_VP, _PI, ctypes.c_int, _PI]),
+    "hz_plan_set_diag_tile": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),
     "hz_host_eigen": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD]),
     "hz_plan_profile": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_plan_profile_read": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI, _PI, ctypes.c_char_p, ctypes.c_int]),
     "hz_last_error_string": (ctypes.c_char_p, []),
     "hz_version": (ctypes.c_char_p, []),
+    "hz_probe_ceilings": (ctypes.c_int, [_PD, _PD, ctypes.c_size_t]),
 }
 
 _lib = None

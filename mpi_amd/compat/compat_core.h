@@ -1,5 +1,5 @@
 // compat_core.h -- the engine shared by the drop-in solvers
-// (NonPeriodic3D.cpp, Periodic3D.cpp, NonPeriodic2D.cpp, NonPeriodic1D.cpp):
+// (Helmholtz3D.cpp, Helmholtz2D.cpp, Helmholtz1D.cpp, each built periodic and not):
 // error handling, the operator recorder behind LUDecomposition, and one GPU
 // plan with pinned staging in the plan's layout.  Internal to the drop-in
 // libraries; the reference-named functions (LUDecomposition, LUSolve,

@@ -41,7 +41,7 @@ struct BatchProblem {
     DiagArgs d;
     int ly, lx;      // transform lengths along y, x
     int ntask[kStages];
-    int dkind;       // 2 * (NJ == 8) + general
+    int dkind;       // 2 * (NP / 32 - 1) + general, NP = 32, 64, 96, 128
 };
 
 namespace {
@@ -72,14 +72,24 @@ __device__ __noinline__ void fft_rows(const FftArgs &a, int L, int batch, double
     }
 }
 
+template <int NJ, int RT, bool GENERAL>
+__device__ __forceinline__ void diag_run_t(const DiagArgs &a, int tile, double *smem)
+{
+    diag_tile<NJ, RT, GENERAL, false>(a, tile, *reinterpret_cast<DiagShared<NJ, RT, GENERAL, false> *>(smem));
+}
+
 template <int RT>
 __device__ __noinline__ void diag_run(const DiagArgs &a, int kind, int tile, double *smem)
 {
     switch (kind) {
-    case 0: diag_tile<4, RT, false, false>(a, tile, *reinterpret_cast<DiagShared<4, RT, false, false> *>(smem)); break;
-    case 1: diag_tile<4, RT, true, false>(a, tile, *reinterpret_cast<DiagShared<4, RT, true, false> *>(smem)); break;
-    case 2: diag_tile<8, RT, false, false>(a, tile, *reinterpret_cast<DiagShared<8, RT, false, false> *>(smem)); break;
-    case 3: diag_tile<8, RT, true, false>(a, tile, *reinterpret_cast<DiagShared<8, RT, true, false> *>(smem)); break;
+    case 0: diag_run_t<2, RT, false>(a, tile, smem); break;
+    case 1: diag_run_t<2, RT, true>(a, tile, smem); break;
+    case 2: diag_run_t<4, RT, false>(a, tile, smem); break;
+    case 3: diag_run_t<4, RT, true>(a, tile, smem); break;
+    case 4: diag_run_t<6, RT, false>(a, tile, smem); break;
+    case 5: diag_run_t<6, RT, true>(a, tile, smem); break;
+    case 6: diag_run_t<8, RT, false>(a, tile, smem); break;
+    case 7: diag_run_t<8, RT, true>(a, tile, smem); break;
     default: break;
     }
 }
@@ -308,7 +318,7 @@ int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, 
         q.lx = pl->nx;
         for (int s = 0; s < kStages; ++s) q.ntask[s] = counts[p][s];
         const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0;
-        q.dkind = 2 * (pl->n > 64) + (general ? 1 : 0);
+        q.dkind = 2 * (diag_padded(pl->n) / 32 - 1) + (general ? 1 : 0);
     }
     return HZ_OK;
 }
@@ -334,11 +344,16 @@ void launch_fft_list(hipStream_t s, const BatchProblem *P, const int2 *list, int
 template <int RT>
 void launch_diag_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int kind)
 {
+    const dim3 g(count), b(256);
     switch (kind) {
-    case 0: hipLaunchKernelGGL((batch_diag_list<4, RT, false>), dim3(count), dim3(256), 0, s, P, list); break;
-    case 1: hipLaunchKernelGGL((batch_diag_list<4, RT, true>), dim3(count), dim3(256), 0, s, P, list); break;
-    case 2: hipLaunchKernelGGL((batch_diag_list<8, RT, false>), dim3(count), dim3(256), 0, s, P, list); break;
-    default: hipLaunchKernelGGL((batch_diag_list<8, RT, true>), dim3(count), dim3(256), 0, s, P, list); break;
+    case 0: hipLaunchKernelGGL((batch_diag_list<2, RT, false>), g, b, 0, s, P, list); break;
+    case 1: hipLaunchKernelGGL((batch_diag_list<2, RT, true>), g, b, 0, s, P, list); break;
+    case 2: hipLaunchKernelGGL((batch_diag_list<4, RT, false>), g, b, 0, s, P, list); break;
+    case 3: hipLaunchKernelGGL((batch_diag_list<4, RT, true>), g, b, 0, s, P, list); break;
+    case 4: hipLaunchKernelGGL((batch_diag_list<6, RT, false>), g, b, 0, s, P, list); break;
+    case 5: hipLaunchKernelGGL((batch_diag_list<6, RT, true>), g, b, 0, s, P, list); break;
+    case 6: hipLaunchKernelGGL((batch_diag_list<8, RT, false>), g, b, 0, s, P, list); break;
+    default: hipLaunchKernelGGL((batch_diag_list<8, RT, true>), g, b, 0, s, P, list); break;
     }
 }
 

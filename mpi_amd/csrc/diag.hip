@@ -37,8 +37,8 @@ int launch_t(const DiagArgs &a, hipStream_t s)
 
 int diag_padded(int n)
 {
-    if (n <= 64) return 64;
-    if (n <= 128) return 128;
+    if (n <= 0) return 0;
+    if (n <= 128) return 32 * ((n + 31) / 32);
     if (n <= 256) return 256;
     return 0;
 }
@@ -47,10 +47,20 @@ int diag_tile_choices(int NP, int *rt, int max)
 {
     // 16 rows: more work-groups in flight to hide the staging latency;
     // 128 rows (N-2 <= 64 only) holds a column of up to 64 elements (82 KB
-    // of LDS: one work-group per CU)
-    static const int c64[] = {16, 32, 64, 128}, c128[] = {16, 32, 48, 64}, c256[] = {16, 32};
-    const int *c = NP == 64 ? c64 : NP == 128 ? c128 : c256;
-    const int k = NP == 256 ? 2 : 4;
+    // of LDS: one work-group per CU).  NP = 32, 96 split the row blocks
+    // between wave pairs, so their tiles have an even number of 16-row blocks.
+    static const int c32[] = {32, 64, 128}, c64[] = {16, 32, 64, 128}, c96[] = {32, 64},
+                     c128[] = {16, 32, 48, 64}, c256[] = {16, 32};
+    const int *c = nullptr;
+    int k = 0;
+    switch (NP) {
+    case 32: c = c32, k = 3; break;
+    case 64: c = c64, k = 4; break;
+    case 96: c = c96, k = 2; break;
+    case 128: c = c128, k = 4; break;
+    case 256: c = c256, k = 2; break;
+    default: return 0;
+    }
     for (int i = 0; i < k && i < max; ++i) rt[i] = c[i];
     return k;
 }
@@ -58,11 +68,14 @@ int diag_tile_choices(int NP, int *rt, int max)
 int launch_diag_modes(const DiagArgs &a, hipStream_t s)
 {
     switch (diag_padded(a.n)) {
+    case 32:
+        return a.rt == 128 ? launch_t<2, 128>(a, s) : a.rt == 64 ? launch_t<2, 64>(a, s) : launch_t<2, 32>(a, s);
     case 64:
         return a.rt == 16   ? launch_t<4, 16>(a, s)
                : a.rt == 32  ? launch_t<4, 32>(a, s)
                : a.rt == 128 ? launch_t<4, 128>(a, s)
                              : launch_t<4, 64>(a, s);
+    case 96: return a.rt == 64 ? launch_t<6, 64>(a, s) : launch_t<6, 32>(a, s);
     case 128:
         return a.rt == 16  ? launch_t<8, 16>(a, s)
                : a.rt == 32 ? launch_t<8, 32>(a, s)

@@ -31,7 +31,9 @@
 //   Y^T = F^T Vinv^T,   U^T = Z^T V^T
 // v_mfma_f64_16x16x4f64 operand layout (wave64, measured on gfx950):
 // A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] (4 per lane) in
-// lane j + 16*(i%4), element i/4.  Each wave computes NJ/4 output column tiles for all RT rows,
+// lane j + 16*(i%4), element i/4.  NP is padded to a multiple of 32; the
+// four waves split the output tile's column tiles four ways (NJ = NP/16 a
+// multiple of 4) or two ways and its row blocks two ways (NJ = 2, 6), each
 // reading A from the LDS tile and B (Vinv^T, V^T) from L2.
 #pragma once
 
@@ -46,17 +48,18 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// acc[rb][jt] += A B over the k index, A = the LDS tile (rows 16 rb + i,
-// column k), B[k][j] = B[k * NP + j] from L2 (j = the wave's column tiles).
-// B is loaded two k-steps ahead so its L2 latency overlaps the MFMAs of the
-// steps in between.
+// acc[rb][jt] += A B over the k index, A = the LDS tile (rows
+// 16 (r0 + rb) + i, column k), B[k][j] = B[k * NP + j] from L2 (j in the
+// column tiles c0 .. c0 + JT - 1).  B is loaded two k-steps ahead so its L2
+// latency overlaps the MFMAs of the steps in between.
 template <int NP, int LD, int RBK, int JT>
 __device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile, const double *__restrict__ B,
-                                          int w, int li, int lk)
+                                          int c0, int r0, int li, int lk)
 {
     constexpr int NK = NP / 4;
     static_assert(NK % 2 == 0, "two k-steps per iteration");
-    const double *bcol = B + 16 * (w * JT) + li + (long long)lk * NP;
+    const double *bcol = B + 16 * c0 + li + (long long)lk * NP;
+    tile += 16 * r0 * LD;
     double b0[JT], b1[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
@@ -115,8 +118,13 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 {
     constexpr int NP = 16 * NJ;      // padded interior size
     constexpr int LD = NP + 2;       // LDS row stride (doubles)
-    constexpr int JT = NJ / 4;       // output column tiles per wave
-    constexpr int RBK = RT / 16;     // row blocks
+    // The four waves split the output's NJ column tiles WC ways and its RBK
+    // row blocks WR ways; each wave computes JT column tiles of RBW row blocks.
+    constexpr int WC = NJ % 4 == 0 ? 4 : 2, WR = 4 / WC;
+    constexpr int JT = NJ / WC;
+    constexpr int RBK = RT / 16;
+    constexpr int RBW = RBK / WR;
+    static_assert(NJ % WC == 0 && RBK % WR == 0, "the waves split the tile evenly");
     double *tile = sh.tile;
     auto &row_sig = sh.row_sig;
     auto &row_zl = sh.row_zl;
@@ -131,6 +139,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     __syncthreads();
 
     const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
+    const int c0 = (w % WC) * JT, r0 = (w / WC) * RBW;  // the wave's first column tile, row block
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
     const int rows_per_fc = 2 * E;                    // re/im x elements
     const int fc_per_tile = RT / rows_per_fc;
@@ -186,13 +195,13 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     __syncthreads();
 
     // ---- Y^T = F^T Vinv^T ----
-    d4 acc[RBK][JT];
+    d4 acc[RBW][JT];
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
+    for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
     const int li = lane % 16, lk = lane / 16;
-    tile_gemm<NP, LD, RBK, JT>(acc, tile, a.VinvT, w, li, lk);
+    tile_gemm<NP, LD, RBW, JT>(acc, tile, a.VinvT, c0, r0, li, lk);
     __syncthreads();  // every wave has read the F tile
 
     // ---- shift, interface sweep, assemble Z^T in the tile ----
@@ -201,20 +210,20 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     // w0|wN . q (the adjoint load on zl, zr).  Each wave reduces its columns
     // over the 16 lanes of a row, then the four waves' partials are summed.
     // One row block at a time, so that only its four partial pairs are live.
-    double inv[RBK][JT][4];
+    double inv[RBW][JT][4];
     const double *dv0 = ADJ ? a.w0 : a.r0, *dvN = ADJ ? a.wN : a.rN;
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb) {
+    for (int rb = 0; rb < RBW; ++rb) {
         double p0[4] = {0.0, 0.0, 0.0, 0.0}, pN[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
-            const int j = 16 * (w * JT + jt) + li;
+            const int j = 16 * (c0 + jt) + li;
             const double mu = j < n ? a.mu[j] : 0.0;
             const double v0 = (GENERAL || ADJ) && j < n ? dv0[j] : 0.0;
             const double vN = (GENERAL || ADJ) && j < n ? dvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
+                const int row = 16 * (r0 + rb) + 4 * r + lk;
                 const double d = mu - row_sig[row];
                 const double iv = (j < n) ? 1.0 / d : 0.0;
                 inv[rb][jt][r] = iv;
@@ -234,7 +243,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                     pN[r] += __shfl_xor(pN[r], o);
                 }
                 if (li == 0) {
-                    const int row = 16 * rb + 4 * r + lk;
+                    const int row = 16 * (r0 + rb) + 4 * r + lk;
                     part0[w][row] = p0[r];
                     partN[w][row] = pN[r];
                 }
@@ -245,10 +254,11 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
         __syncthreads();
         for (int row = tid; row < RT; row += 256) {
             double s0 = 0.0, sN = 0.0;
+            const int wbase = (row / 16 / RBW) * WC;  // the waves that cover the row
 #pragma unroll
-            for (int ww = 0; ww < 4; ++ww) {
-                s0 += part0[ww][row];
-                sN += partN[ww][row];
+            for (int c = 0; c < WC; ++c) {
+                s0 += part0[wbase + c][row];
+                sN += partN[wbase + c][row];
             }
             row_d0[row] = s0;
             row_dN[row] = sN;
@@ -302,34 +312,8 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                 continue;
             }
             const double *bsrc = plane ? a.bc_im : a.bc_re;
-            const double gl = bsrc[fc], gr = bsrc[(long long)nfc + fc];
-            double bprev = -a.alpha_l * d0ul + a.beta_l;
-            double cprev = -a.alpha_l * d0ur;
-            double rprev = gl + a.alpha_l * row_d0[r0w];
-            row_zl[r0w] = bprev;  // b, rhs of the forward elimination kept in zl/zr
-            row_zr[r0w] = rprev;
-            for (int e = 1; e < E; ++e) {
-                const double lb = dNul;
-                const double be = dNur - d0ul - (cprev / bprev) * lb;
-                const double re = row_d0[r0w + e] - row_dN[r0w + e - 1] - (rprev / bprev) * lb;
-                bprev = be;
-                cprev = -d0ur;
-                rprev = re;
-                row_zl[r0w + e] = be;
-                row_zr[r0w + e] = re;
-            }
-            const double lb = a.alpha_r * dNul;
-            const double br = a.alpha_r * dNur + a.beta_r - (cprev / bprev) * lb;
-            const double rhsr = gr - a.alpha_r * row_dN[r0w + E - 1] - (rprev / bprev) * lb;
-            double zr = rhsr / br;
-            for (int e = E - 1; e >= 0; --e) {
-                const double be = row_zl[r0w + e], re = row_zr[r0w + e];
-                const double ce = (e == 0) ? -a.alpha_l * d0ur : -d0ur;
-                const double zl = (re - ce * zr) / be;
-                row_zl[r0w + e] = zl;
-                row_zr[r0w + e] = zr;
-                zr = zl;
-            }
+            robin_sweep(E, d0ul, dNul, d0ur, dNur, bsrc[fc], bsrc[(long long)nfc + fc], a.alpha_l, a.beta_l,
+                        a.alpha_r, a.beta_r, row_d0 + r0w, row_dN + r0w, row_zl + r0w, row_zr + r0w);
         }
     } else if constexpr (!ADJ) {
         // one element, Dirichlet data on both faces: zl = g_l, zr = g_r
@@ -349,14 +333,14 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     __syncthreads();
     const double *zv0 = ADJ ? a.r0 : a.w0, *zvN = ADJ ? a.rN : a.wN;
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
+    for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
-            const int j = 16 * (w * JT + jt) + li;
+            const int j = 16 * (c0 + jt) + li;
             const double w0 = j < n ? zv0[j] : 0.0, wN = j < n ? zvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
+                const int row = 16 * (r0 + rb) + 4 * r + lk;
                 const double h = fma(row_zl[row], w0, row_zr[row] * wN);
                 tile[row * LD + j] = (acc[rb][jt][r] + h) * inv[rb][jt][r];
             }
@@ -365,18 +349,18 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 
     // ---- U^T = Z^T V^T, stored along z ----
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
+    for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
-    tile_gemm<NP, LD, RBK, JT>(acc, tile, a.VT, w, li, lk);
+    tile_gemm<NP, LD, RBW, JT>(acc, tile, a.VT, c0, r0, li, lk);
 #pragma unroll
-    for (int rb = 0; rb < RBK; ++rb)
+    for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
-            const int k = 16 * (w * JT + jt) + li;
+            const int k = 16 * (c0 + jt) + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + 4 * r + lk;
+                const int row = 16 * (r0 + rb) + 4 * r + lk;
                 const long long col = row_col[row];
                 if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
             }

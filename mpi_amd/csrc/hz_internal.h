@@ -93,7 +93,7 @@ struct DiagArgs {
     const double *bc_re, *bc_im;
     const double *shift;        // [modes]
     const double *mu, *w0, *wN, *r0, *rN;  // [n]
-    const double *VinvT, *VT;   // [NP][NP], NP = diag_padded(n)
+    const double *VinvT, *VT;   // [NP][NP], NP = diag_padded(n): n rounded up to 32 (256 above 128)
     const double *hom;          // [modes][4]: Dz0.ul, DzN.ul, Dz0.ur, DzN.ur
     double alpha_l, beta_l, alpha_r, beta_r;
     int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
@@ -105,6 +105,7 @@ int diag_padded(int n);
 // Tile heights offered for padded size NP (autotuned per plan).
 int diag_tile_choices(int NP, int *rt, int max);
 int launch_diag_modes(const DiagArgs &a, hipStream_t s);
+
 
 // MIXED form (mixed.hip).
 struct MixedSetupArgs {

@@ -432,8 +432,8 @@ namespace {
 // (zeroed) spectrum and the fastest kept.
 int tune_diag(hz_plan *p)
 {
-    int choices[4];
-    const int k = diag_tile_choices(p->NP, choices, 4);
+    int choices[8];
+    const int k = diag_tile_choices(p->NP, choices, 8);
     HZ_HIP(hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->M * p->K));
     HZ_HIP(hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M));
     hipStream_t s;
@@ -893,6 +893,30 @@ int hz_plan_tuning(const hz_plan *p, int *diag_tile_rows, double *diag_ms, int m
         if (fft_lanes) fft_lanes[pass] = p->fft_w[pass];
         if (fft_ms) fft_ms[pass] = p->fft_w_ms[pass];
     }
+    return HZ_OK;
+}
+
+int hz_plan_diag_candidates(const hz_plan *p, int *tile_rows, int max, int *count)
+{
+    if (!p || !count) return set_error(HZ_ERR_ARG, "null argument");
+    int c[8];
+    const int k = p->d.form == HZ_FORM_DIAG ? diag_tile_choices(p->NP, c, 8) : 0;
+    for (int i = 0; i < k && i < max && tile_rows; ++i) tile_rows[i] = c[i];
+    *count = k;
+    return HZ_OK;
+}
+
+int hz_plan_set_diag_tile(hz_plan *p, int tile_rows)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "not a DIAG plan");
+    int c[8];
+    const int k = diag_tile_choices(p->NP, c, 8);
+    bool known = false;
+    for (int i = 0; i < k; ++i) known = known || c[i] == tile_rows;
+    if (!known) return set_error(HZ_ERR_ARG, "not a candidate tile shape for this plan");
+    if (tile_rows < 2 * p->ne) return set_error(HZ_ERR_ARG, "the tile does not hold one column's elements");
+    p->diag_rt = tile_rows;
     return HZ_OK;
 }
 

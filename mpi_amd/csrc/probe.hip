@@ -1,0 +1,119 @@
+// probe.hip -- attainable ceilings of the device, measured live: the FP64
+// MFMA rate of v_mfma_f64_16x16x4f64 (the instruction of the mode solve)
+// and the bandwidth of a plain device-to-device copy.  The published peaks
+// (78.6 TFLOP/s dense FP64 matrix, 8 TB/s HBM3E) are upper bounds that no
+// kernel of this shape reaches; the roofline fractions of the solver's
+// kernels are reported against both.
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Two independent accumulation chains per wave, eight waves per SIMD: the
+// best shape found for this instruction (one chain or fewer waves issue
+// less often).  The operands are lane-dependent so nothing folds.
+__global__ __launch_bounds__(256) void mfma_f64_probe(double *out, int iters)
+{
+    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+    const double a = 1.0 + threadIdx.x * 1e-9, b = 0.5 - threadIdx.x * 1e-9;
+    for (int it = 0; it < iters; ++it) {
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc1, 0, 0, 0);
+    }
+    out[(long long)blockIdx.x * blockDim.x + threadIdx.x] =
+        acc0[0] + acc0[1] + acc0[2] + acc0[3] + acc1[0] + acc1[1] + acc1[2] + acc1[3];
+}
+
+// One 16-byte element per thread, the grid covering the buffer: the
+// fastest of the copy shapes measured (grid-stride loops and several
+// elements per thread ran 5-25% slower).
+__global__ __launch_bounds__(256) void copy_probe(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                  long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+// Mean milliseconds of `reps` back-to-back launches of `launch` on stream s,
+// after two untimed ones.
+template <class F>
+int time_ms(hipStream_t s, F launch, float *ms, int reps)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return set_error(HZ_ERR_HIP, "event creation failed");
+    }
+    launch();
+    launch();
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, s);
+    int st = HZ_OK;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        st = set_error(HZ_ERR_HIP, "probe run failed");
+    *ms /= reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+}  // namespace
+}  // namespace hz
+
+using namespace hz;
+
+int hz_probe_ceilings(double *fp64_mfma_tflops, double *copy_gbps, size_t copy_bytes)
+{
+    if (!fp64_mfma_tflops && !copy_gbps) return set_error(HZ_ERR_ARG, "null argument");
+    int dev = 0, cus = 0;
+    HZ_HIP(hipGetDevice(&dev));
+    HZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipStream_t s = nullptr;
+    HZ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int st = HZ_OK;
+    if (fp64_mfma_tflops) {
+        const int wgs = 8 * cus, iters = 4096;  // eight waves per SIMD
+        double *out = nullptr;
+        if (hipMalloc((void **)&out, sizeof(double) * 256 * wgs) != hipSuccess) {
+            st = set_error(HZ_ERR_ALLOC, "probe allocation failed");
+        } else {
+            float ms = 0.0f;
+            st = time_ms(
+                s, [&] { hipLaunchKernelGGL(mfma_f64_probe, dim3(wgs), dim3(256), 0, s, out, iters); }, &ms, 1);
+            // 4 waves x 2 chains x iters MFMAs of 16 x 16 x 4 multiply-adds per work-group
+            if (!st) *fp64_mfma_tflops = (double)wgs * 4 * 2 * iters * 2048.0 / (ms * 1e-3) / 1e12;
+            (void)hipFree(out);
+        }
+    }
+    if (!st && copy_gbps) {
+        const long long n = (long long)(copy_bytes / sizeof(double2));  // elements of each buffer
+        double2 *a = nullptr, *b = nullptr;
+        if (n < 1) {
+            st = set_error(HZ_ERR_ARG, "copy size too small");
+        } else if (hipMalloc((void **)&a, sizeof(double2) * n) != hipSuccess ||
+                   hipMalloc((void **)&b, sizeof(double2) * n) != hipSuccess) {
+            st = set_error(HZ_ERR_ALLOC, "probe allocation failed");
+        } else if (hipMemsetAsync(a, 0, sizeof(double2) * n, s) != hipSuccess) {
+            st = set_error(HZ_ERR_HIP, "probe buffer clear failed");
+        } else {
+            float ms = 0.0f;
+            const long long grid = (n + 255) / 256;
+            if (grid > 0x7fffffffLL) {
+                st = set_error(HZ_ERR_ARG, "copy size too large");
+            } else {
+                st = time_ms(
+                    s, [&] { hipLaunchKernelGGL(copy_probe, dim3((unsigned)grid), dim3(256), 0, s, a, b, n); }, &ms,
+                    10);
+            }
+            if (!st) *copy_gbps = 2.0 * sizeof(double2) * n / (ms * 1e-3) / 1e9;  // read + write
+        }
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    }
+    (void)hipStreamDestroy(s);
+    return st;
+}

@@ -68,5 +68,44 @@ __device__ __forceinline__ void periodic_sweep(int E, double d0ul, double dNul, 
     }
 }
 
+// Element-interface system of one real column with Robin faces
+// beta_l u - alpha_l u_z = gl, beta_r u + alpha_r u_z = gr
+// (NonPeriodic3D_MPI.cpp:327-587 at one rank): forward elimination over the
+// elements, then back substitution.  d0, dN, zl, zr as in periodic_sweep
+// (element e at [e]); zl, zr carry b and the right-hand side through the
+// forward sweep.
+__device__ __forceinline__ void robin_sweep(int E, double d0ul, double dNul, double d0ur, double dNur, double gl,
+                                            double gr, double al, double bl, double ar, double br,
+                                            const double *d0, const double *dN, double *zl, double *zr)
+{
+    double bprev = -al * d0ul + bl;
+    double cprev = -al * d0ur;
+    double rprev = gl + al * d0[0];
+    zl[0] = bprev;
+    zr[0] = rprev;
+    for (int e = 1; e < E; ++e) {
+        const double lb = dNul;
+        const double be = dNur - d0ul - (cprev / bprev) * lb;
+        const double re = d0[e] - dN[e - 1] - (rprev / bprev) * lb;
+        bprev = be;
+        cprev = -d0ur;
+        rprev = re;
+        zl[e] = be;
+        zr[e] = re;
+    }
+    const double lb = ar * dNul;
+    const double bE = ar * dNur + br - (cprev / bprev) * lb;
+    const double rhsr = gr - ar * dN[E - 1] - (rprev / bprev) * lb;
+    double z = rhsr / bE;
+    for (int e = E - 1; e >= 0; --e) {
+        const double be = zl[e], re = zr[e];
+        const double ce = (e == 0) ? -al * d0ur : -d0ur;
+        const double zle = (re - ce * z) / be;
+        zl[e] = zle;
+        zr[e] = z;
+        z = zle;
+    }
+}
+
 }  // namespace
 }  // namespace hz

@@ -171,13 +171,26 @@ class HelmholtzPlan:
         z lanes per work-group of the four transform passes (and their ms)."""
         P = ctypes.POINTER(ctypes.c_double)
         rt = ctypes.c_int(0)
-        ms = np.zeros(4)
+        ms = np.zeros(8)
         lanes = (ctypes.c_int * 4)()
         fms = np.zeros(4)
-        check(self.lib.hz_plan_tuning(self.handle, ctypes.byref(rt), ms.ctypes.data_as(P), 4, lanes,
+        check(self.lib.hz_plan_tuning(self.handle, ctypes.byref(rt), ms.ctypes.data_as(P), 8, lanes,
                                       fms.ctypes.data_as(P)))
+        cand = self.diag_candidates()
         return {"diag_tile_rows": rt.value, "choice_ms": [v for v in ms.tolist() if v > 0],
+                "choices": {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0},
                 "fft_lanes": list(lanes), "fft_ms": fms.tolist()}
+
+    def diag_candidates(self):
+        """Tile shapes (rows per tile) the DIAG mode kernel offers for this plan."""
+        rows = (ctypes.c_int * 8)()
+        k = ctypes.c_int(0)
+        check(self.lib.hz_plan_diag_candidates(self.handle, rows, 8, ctypes.byref(k)))
+        return list(rows)[:k.value]
+
+    def set_diag_tile(self, rows):
+        """Pins the DIAG mode kernel's shape to one of diag_candidates()."""
+        check(self.lib.hz_plan_set_diag_tile(self.handle, int(rows)))
 
     def mixed_fp64_modes(self):
         k = ctypes.c_int(0)

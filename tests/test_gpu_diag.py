@@ -92,7 +92,8 @@ def test_diag_plan_autotune(cuda, N):
     plan = HelmholtzPlan(64, 16, N, 1, lam=2.0, form="diag")
     t = plan.tuning()
     print(f"N={N}: {t}")
-    assert t["diag_tile_rows"] in {16, 32, 48, 64, 128} and len(t["choice_ms"]) >= 2
+    assert t["diag_tile_rows"] in set(plan.diag_candidates()) and len(t["choice_ms"]) >= 2
+    assert set(t["choices"]) <= set(plan.diag_candidates())
     # nx = 64 offers three lane widths (passes 1, 2), ny = 16 one shape (passes 0, 3)
     assert t["fft_lanes"][0] == 0 and t["fft_lanes"][3] == 0
     assert t["fft_lanes"][1] in {8, 16, 32} and t["fft_lanes"][2] in {8, 16, 32}
@@ -103,3 +104,69 @@ def test_diag_plan_autotune(cuda, N):
     u = plan.solve(to_dev(f), to_dev(g), to_dev(g)).cpu().numpy()
     assert rel_err(u, po.solve(desc, f, g, g)) <= TOL
     plan.close()
+
+
+# N-2 is padded to a multiple of 32 for the FP64 MFMA contractions: 15, 31
+# -> 32 (the waves split columns and row blocks two ways), 63 -> 64, 95 -> 96
+# (two ways again), 127 -> 128.  (ne, robin): the plain Dirichlet tile and the
+# general one with the element sweep.
+@pytest.mark.parametrize("N", [17, 33, 65, 97, 129])
+@pytest.mark.parametrize("ne,robin", [(1, False), (2, True)])
+def test_diag_every_tile_shape(cuda, N, ne, robin):
+    """Every candidate tile shape gives bit for bit the same solve (each
+    output element is the same sequence of operations whatever the tile
+    height), and that solve matches the oracle."""
+    import torch
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny = 16, 8
+    al, bl, ar, br = (0.6, 1.2, 0.3, 0.9) if robin else (0.0, 1.0, 0.0, 1.0)
+    plan = HelmholtzPlan(nx, ny, N, ne, lam=3.0, alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form="diag")
+    rng = np.random.default_rng(N + 7 * ne)
+    f = rng.standard_normal((nx, ny, plan.nz))
+    gl = rng.standard_normal((nx, ny))
+    gr = rng.standard_normal((nx, ny))
+    shapes = [rt for rt in plan.diag_candidates() if rt >= 2 * ne]
+    assert len(shapes) >= 2
+    outs = []
+    for rt in shapes:
+        plan.set_diag_tile(rt)
+        outs.append(plan.solve(to_dev(f), to_dev(gl), to_dev(gr)))
+    for rt, u in zip(shapes[1:], outs[1:]):
+        assert torch.equal(u, outs[0]), f"tile {rt} differs from tile {shapes[0]}"
+    desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
+                        alphal=al, betal=bl, alphar=ar, betar=br, lam=3.0)
+    assert rel_err(outs[0].cpu().numpy(), po.solve(desc, f, gl, gr)) <= TOL
+    plan.close()
+
+
+def test_diag_tile_pinning_rules(cuda):
+    """Only the plan's candidate shapes that hold a column's element rows are
+    accepted."""
+    from mpi_amd._lib import HzError
+    from mpi_amd.solver import HelmholtzPlan
+    small = HelmholtzPlan(16, 16, 33, 1, lam=1.0, form="diag")
+    assert small.diag_candidates() == [32, 64, 128]
+    for bad in (16, 24, -32):
+        with pytest.raises(HzError):
+            small.set_diag_tile(bad)
+    small.close()
+    many = HelmholtzPlan(16, 16, 129, 20, lam=1.0, form="diag")
+    assert 32 in many.diag_candidates()
+    with pytest.raises(HzError):
+        many.set_diag_tile(32)  # 40 element rows per column
+    many.set_diag_tile(48)
+    many.close()
+
+
+def test_probe_ceilings_are_plausible(cuda):
+    """hz_probe_ceilings measures the FP64 MFMA and copy rates the roofline
+    fractions are also quoted against: within (0, published peak]."""
+    import ctypes
+    from mpi_amd import _lib
+    lib = _lib.load()
+    mfma, copy = ctypes.c_double(0), ctypes.c_double(0)
+    _lib.check(lib.hz_probe_ceilings(ctypes.byref(mfma), ctypes.byref(copy), 64 << 20))
+    assert 5.0 < mfma.value <= 78.6 * 1.02
+    assert 300.0 < copy.value <= 8000.0 * 1.5  # an Infinity-Cache-resident copy can beat HBM
+    with pytest.raises(_lib.HzError):
+        _lib.check(lib.hz_probe_ceilings(None, None, 1 << 20))
