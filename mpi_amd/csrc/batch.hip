@@ -4,9 +4,10 @@
 // Every problem is the five-stage pipeline of a solve:
 //   0 R2C along y (+ faces)   1 C2C along x (+ faces)   2 diagonalised mode solve
 //   3 C2C^-1 along x          4 C2R along y
-// split into tasks: one line batch of a transform pass (all its z-lane
-// blocks, so no two work-groups share the cache lines of one row), or one
-// tile of spectral columns of the mode solve.
+// split into tasks: a tile of spectral columns of the mode solve, or for a
+// transform pass one uniform-base tile of the standalone transform kernels
+// (staged variant) or one whole line batch with all its z-lane blocks
+// (fused variant).
 //
 // Fused variant (built, switched off -- see kVariants): ONE persistent
 // launch.  Tasks are numbered stage-major
