@@ -248,7 +248,7 @@ def time_allen_cahn():
             "loss_and_grad_ms": 1e3 * (t2 - t1), "loss": L, "dL_deps": deps}
 
 
-def time_batch(steps=5):
+def time_batch(steps=20):
     """Many-problem workload: 512 independent problems, nx, ny uniform in
     {32, 64, 128}, N odd uniform in [17, 129], lambda log-uniform in
     [1e-2, 1e3], 16 random modes each; one hz_batch (autotuned, graph

@@ -96,6 +96,10 @@ def load(path=LIB_PATH):
         if not os.path.exists(path):
             raise ImportError(f"{path} is not built; run __graft_entry__.build() "
                               "(make -C mpi_amd/csrc)")
+        try:  # torch ships its own HIP runtime: load it first so that libhz binds to the same one
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
