@@ -318,7 +318,7 @@ int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, 
         q.ly = pl->ny;
         q.lx = pl->nx;
         for (int s = 0; s < kStages; ++s) q.ntask[s] = counts[p][s];
-        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0;
+        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
         q.dkind = 2 * (diag_padded(pl->n) / 32 - 1) + (general ? 1 : 0);
     }
     return HZ_OK;
@@ -409,10 +409,13 @@ int build_lists(hz_batch *b, int rt)
             if (st != 2) {  // transform passes: one entry per uniform-base tile
                 // The table is filled with null buffers here, so the face
                 // count is set from what hz_batch_solve always passes: both
-                // faces in the forward passes (its gl, gr are never null),
-                // none in the inverse ones.  Face offsets are below nx*M.
+                // faces in the forward passes (its gl, gr are never null for
+                // a problem with faces), none in the inverse ones, none for
+                // periodic z.  Face offsets are below nx*M.
                 const FftArgs &a = b->h_probs[p].f[st < 2 ? st : st - 1];
-                const int L = task_kind(b->h_probs[p], st), W = batch_lanes(L), faces = st < 2 ? 2 : 0;
+                const bool has_faces = !b->plans[p]->d.periodic_z;
+                const int L = task_kind(b->h_probs[p], st), W = batch_lanes(L);
+                const int faces = st < 2 && has_faces ? 2 : 0;
                 const long long lim = 1LL << 29;  // element offsets whose byte offsets fit 32 bits
                 const long long m = (W - 1) + (long long)(L - 1) * std::max(a.in_point_stride, a.out_point_stride);
                 if (m >= lim) return set_error(HZ_ERR_UNSUPPORTED, "batch: problem too large for a tile");
@@ -487,15 +490,14 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         for (int q = 0; q < p; ++q)
             if (plans[q] == pl) return set_error(HZ_ERR_ARG, "a plan appears twice in the batch");
         if (pl->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG-form plans only");
-        if (pl->d.periodic_z) return set_error(HZ_ERR_UNSUPPORTED, "batch: periodic z not supported");
         if (pl->device != dev) return set_error(HZ_ERR_ARG, "batch: plans on another device");
         if (pl->n > 128) return set_error(HZ_ERR_UNSUPPORTED, "batch: N-2 must be <= 128");
         if (pl->nx < 16 || pl->nx > 256 || pl->ny < 16 || pl->ny > 256)
             return set_error(HZ_ERR_UNSUPPORTED, "batch: nx, ny must be in [16, 256]");
         if (2 * pl->ne > 32) return set_error(HZ_ERR_UNSUPPORTED, "batch: at most 16 elements");
-        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0;
+        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
         shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +
-                 std::to_string(pl->ne) + (general ? "g;" : ";");
+                 std::to_string(pl->ne) + (pl->d.periodic_z ? "p;" : general ? "g;" : ";");
     }
     hz_batch *b = new hz_batch();
     b->plans.assign(plans, plans + count);
@@ -554,8 +556,11 @@ int hz_batch_solve(hz_batch *b, const double *const *f, const double *const *gl,
                    double *const *u, hz_stream_t stream)
 {
     if (!b || !f || !gl || !gr || !u) return set_error(HZ_ERR_ARG, "null argument");
-    for (int p = 0; p < b->n; ++p)
-        if (!f[p] || !gl[p] || !gr[p] || !u[p]) return set_error(HZ_ERR_ARG, "null buffer");
+    for (int p = 0; p < b->n; ++p) {
+        if (!f[p] || !u[p]) return set_error(HZ_ERR_ARG, "null buffer");
+        if (b->plans[p]->d.periodic_z ? (gl[p] || gr[p]) : (!gl[p] || !gr[p]))
+            return set_error(HZ_ERR_ARG, "face data: required without periodic z, NULL with it");
+    }
     const hipStream_t cs = (hipStream_t)stream;
     int st;
     HZ_HIP(hipEventRecord(b->ev_in, cs));

@@ -229,12 +229,13 @@ class BatchSolver:
             us = [p.empty_field() for p in self.plans]
         for p, f, gl, gr, u in zip(self.plans, fs, gls, grs, us):
             p._check_field(f, (p.nx, p.ny, p.nz))
-            p._check_field(gl, (p.nx, p.ny))
-            p._check_field(gr, (p.nx, p.ny))
+            if not p.desc.periodic_z:  # periodic-z problems take None for both faces
+                p._check_field(gl, (p.nx, p.ny))
+                p._check_field(gr, (p.nx, p.ny))
             p._check_field(u, (p.nx, p.ny, p.nz))
         P = ctypes.c_void_p * n
-        check(self.lib.hz_batch_solve(self.handle, P(*[f.data_ptr() for f in fs]), P(*[g.data_ptr() for g in gls]),
-                                      P(*[g.data_ptr() for g in grs]), P(*[u.data_ptr() for u in us]),
+        check(self.lib.hz_batch_solve(self.handle, P(*[f.data_ptr() for f in fs]), P(*[_ptr(g) for g in gls]),
+                                      P(*[_ptr(g) for g in grs]), P(*[u.data_ptr() for u in us]),
                                       _stream_ptr(stream)))
         return us
 

@@ -178,3 +178,47 @@ def test_one_shot_batched_solve(cuda):
         assert rel(u.cpu().numpy(), w) <= 1e-13, k
     for p in plans:
         p.close()
+
+
+# (nx, ny, N, ne, lam): periodic-z problems mixed into a batch with Robin ones
+PERIODIC_SET = [(32, 16, 33, 1, 0.5), (16, 64, 17, 3, 2.0), (64, 32, 65, 2, 7.0), (16, 16, 97, 1, 1e-2)]
+
+
+def test_batch_with_periodic_problems(cuda):
+    """Periodic-z plans (no face data) batched together with Robin/Dirichlet
+    ones: every problem equals its single-plan solve, the periodic ones match
+    the oracle's periodic solve, and face data for a periodic problem is
+    refused."""
+    from mpi_amd._lib import HzError
+    from mpi_amd.solver import BatchSolver, HelmholtzPlan
+    plans, inputs, _ = make_set(MIXED_SET[:3])
+    pdescs = []
+    for k, (nx, ny, N, ne, lam) in enumerate(PERIODIC_SET):
+        p = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam, form="diag",
+                          periodic_z=True)
+        rng = np.random.default_rng(300 + k)
+        plans.append(p)
+        inputs.append((rng.standard_normal((nx, ny, p.nz)), None, None))
+        pdescs.append(po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2,
+                                   lam=lam))
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [None if g is None else dev(g) for _, g, _ in inputs]
+    grs = [None if g is None else dev(g) for _, _, g in inputs]
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    b = BatchSolver(plans)
+    us = b.solve(fs, gls, grs)
+    b.status()
+    for k, (u, w) in enumerate(zip(us, want)):
+        assert rel(u.cpu().numpy(), w) <= 1e-13, f"problem {k}"
+    for k, d in enumerate(pdescs):
+        # Bar: 1e-10, or twice the reference algorithm's own rounding spread
+        # (the oracle built with FMA contraction) where that is larger: the
+        # constant mode of a periodic problem with lambda = 1e-2 is close to
+        # singular, and the two oracle builds differ by 5e-10 there.
+        f = inputs[3 + k][0]
+        uo = po.solve_periodic(d, f)
+        spread = rel(po.solve_periodic(d, f, fma=True), uo)
+        assert rel(us[3 + k].cpu().numpy(), uo) <= max(1e-10, 2 * spread), f"periodic problem {k}"
+    with pytest.raises(HzError):  # face data given for a periodic problem
+        b.solve(fs, gls[:3] + [fs[3][:, :, 0].contiguous()] + gls[4:], grs)
+    b.close()
