@@ -22,7 +22,8 @@ class HelmholtzSolve(torch.autograd.Function):
         lam_v = float(lam)
         if plan.desc.lam != lam_v:
             plan.set_lambda(lam_v)
-        u = plan.solve(f.contiguous(), gl.contiguous(), gr.contiguous())
+        faces = [None if g is None else g.contiguous() for g in (gl, gr)]  # None for periodic z
+        u = plan.solve(f.contiguous(), *faces)
         ctx.plan = plan
         ctx.lam = lam_v
         ctx.lam_like = lam if torch.is_tensor(lam) else None

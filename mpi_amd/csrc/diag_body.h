@@ -266,7 +266,8 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     }
     if constexpr (ADJ) {
         __syncthreads();
-        // transposed sweep (one element): zbar -> gbar, P0bar, PNbar
+        // transposed sweep (one element): zbar -> gbar, P0bar, PNbar (Robin
+        // faces) or the transposed cyclic closure (periodic z)
         for (int row = tid; row < RT; row += 256) {
             const int fl = row >> 1, plane = row & 1;
             const int fc = fc0 + fl;
@@ -277,18 +278,28 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                 const int mode = iw * M + m;
                 const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
                 const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
-                const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
-                const double lb = a.alpha_r * LN;
-                const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
-                const double zlb = row_u0[row] + row_d0[row];
-                const double zrb = row_uN[row] + row_dN[row] - (c / b) * zlb;
-                const double rhsrb = zrb / br;
-                const double rhsb = zlb / b - (lb / b) * rhsrb;
-                P0b = a.alpha_l * rhsb;
-                PNb = -a.alpha_r * rhsrb;
-                double *gre = plane ? a.gbar_im : a.gbar_re;
-                gre[fc] = rhsb;
-                gre[(long long)nfc + fc] = rhsrb;
+                if (a.periodic) {
+                    // u_0 = u_{N-1} = z = (dN - d0) / bp (periodic_sweep, one
+                    // element): zbar collects both end values and the load
+                    // through ul + ur; d0bar = -zbar / bp, dNbar = zbar / bp
+                    const double bp = (L0 + R0) - (LN + RN);
+                    const double zb = (row_u0[row] + row_uN[row]) + (row_d0[row] + row_dN[row]);
+                    P0b = -zb / bp;
+                    PNb = zb / bp;
+                } else {
+                    const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
+                    const double lb = a.alpha_r * LN;
+                    const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
+                    const double zlb = row_u0[row] + row_d0[row];
+                    const double zrb = row_uN[row] + row_dN[row] - (c / b) * zlb;
+                    const double rhsrb = zrb / br;
+                    const double rhsb = zlb / b - (lb / b) * rhsrb;
+                    P0b = a.alpha_l * rhsb;
+                    PNb = -a.alpha_r * rhsrb;
+                    double *gre = plane ? a.gbar_im : a.gbar_re;
+                    gre[fc] = rhsb;
+                    gre[(long long)nfc + fc] = rhsrb;
+                }
             }
             row_zl[row] = P0b;  // reused: P0bar, PNbar per row
             row_zr[row] = PNb;

@@ -832,9 +832,10 @@ int hz_solve_status(hz_plan *p, hz_stream_t stream)
 int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar,
                double *grbar, double *lambda_bar, hz_stream_t stream)
 {
-    if (!p || !ubar || !fbar || !glbar || !grbar) return set_error(HZ_ERR_ARG, "null argument");
+    if (!p || !ubar || !fbar) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->d.periodic_z ? (glbar || grbar) : (!glbar || !grbar))
+        return set_error(HZ_ERR_ARG, "face gradients: required without periodic z, NULL with it");
     if (p->ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: one element only");
-    if (p->d.periodic_z) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: periodic z not supported");
     if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
     const hipStream_t s = (hipStream_t)stream;
     int st;

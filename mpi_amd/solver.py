@@ -103,9 +103,14 @@ class HelmholtzPlan:
         check(self.lib.hz_solve_status(self.handle, _stream_ptr(stream)))
 
     def adjoint(self, ubar, u=None, stream=None):
+        """(dL/df, dL/dg_l, dL/dg_r, dL/dlambda or None) for dL/du = ubar; the
+        face gradients are None for a periodic-z plan."""
         fbar = self.empty_field()
-        glbar = torch.empty((self.nx, self.ny), dtype=torch.float64, device="cuda")
-        grbar = torch.empty_like(glbar)
+        if self.desc.periodic_z:
+            glbar = grbar = None
+        else:
+            glbar = torch.empty((self.nx, self.ny), dtype=torch.float64, device="cuda")
+            grbar = torch.empty_like(glbar)
         lam_bar = ctypes.c_double(0.0)
         check(self.lib.hz_adjoint(self.handle, _ptr(ubar), _ptr(u), _ptr(fbar), _ptr(glbar), _ptr(grbar),
                                   ctypes.byref(lam_bar) if u is not None else None, _stream_ptr(stream)))

@@ -186,3 +186,62 @@ def test_allen_cahn_checkpointing_is_exact(cuda):
     L1, g1, e1 = ac.loss_and_grad(u0, 0.02, target)
     assert L8 == L1 and e8 == e1 and torch.equal(g8, g1)
     ac.close()
+
+
+@pytest.mark.parametrize("form", ["diag", "plain"])
+def test_periodic_adjoint_equals_oracle_transpose(cuda, form):
+    """Periodic z (no face data): the GPU adjoint equals S^T ubar with S the
+    oracle's periodic solve map, column by column; then the dot-product
+    identity and a lambda finite difference at a larger size, and autograd."""
+    import torch
+    from mpi_amd.autograd import helmholtz_solve
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny, N, lam = 4, 8, 9, 3.0
+    desc = po.make_desc(nx, ny, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam)
+    K = po.total_z_points(desc)
+    nf = nx * ny * K
+    S = np.zeros((nf, nf))
+    for c in range(nf):
+        f = np.zeros(nf)
+        f[c] = 1.0
+        S[:, c] = po.solve_periodic(desc, f).ravel()
+    ubar = np.random.default_rng(12).standard_normal((nx, ny, K))
+    want = S.T @ ubar.ravel()
+    plan = HelmholtzPlan(nx, ny, N, 1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam, form=form,
+                         periodic_z=True)
+    fbar, glbar, grbar, _ = plan.adjoint(dev(ubar))
+    assert glbar is None and grbar is None
+    got = fbar.cpu().numpy().ravel()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    plan.close()
+
+    if form != "diag":
+        return
+    nx, N, lam = 64, 65, 5.0
+    plan = HelmholtzPlan(nx, nx, N, 1, lam=lam, form="diag", periodic_z=True)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    f = torch.randn((nx, nx, plan.nz), dtype=torch.float64, device="cuda", generator=g)
+    f[:, :, 0] = 0.0  # the solve ignores f on the end nodes
+    f[:, :, -1] = 0.0
+    ubar = torch.randn((nx, nx, plan.nz), dtype=torch.float64, device="cuda", generator=g)
+    u = plan.solve(f)
+    fbar, _, _, lam_bar = plan.adjoint(ubar, u)
+    lhs, rhs = float(torch.sum(u * ubar)), float(torch.sum(f * fbar))
+    assert abs(lhs - rhs) <= 1e-12 * float(torch.sum(torch.abs(u * ubar)))
+
+    def L(lv):
+        plan.set_lambda(lv)
+        return float(torch.sum(ubar * plan.solve(f)))
+
+    h = 1e-4 * lam
+    fd = (L(lam + h) - L(lam - h)) / (2 * h)
+    plan.set_lambda(lam)
+    assert abs(lam_bar - fd) <= 1e-6 * abs(fd)
+    plan.close()
+
+    small = HelmholtzPlan(4, 4, 7, 1, lam=2.0, form="diag", periodic_z=True)
+    fs = torch.randn((4, 4, small.nz), dtype=torch.float64, device="cuda", generator=g, requires_grad=True)
+    lt = torch.tensor(2.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda a, b: helmholtz_solve(a, None, None, b, small), (fs, lt),
+                                    eps=1e-6, atol=1e-7, rtol=1e-6)
+    small.close()

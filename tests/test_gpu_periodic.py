@@ -70,7 +70,7 @@ def test_periodic_rejects_unsupported(cuda):
     from mpi_amd.solver import HelmholtzPlan
     with pytest.raises(HzError):
         HelmholtzPlan(8, 8, 9, 1, form="mixed", periodic_z=True)
-    p = HelmholtzPlan(8, 8, 9, 1, form="diag", periodic_z=True)
-    with pytest.raises(HzError):
+    p = HelmholtzPlan(8, 8, 9, 2, form="diag", periodic_z=True)
+    with pytest.raises(HzError):  # the adjoint takes one element
         p.adjoint(torch.zeros((8, 8, p.nz), dtype=torch.float64, device="cuda"))
     p.close()
