@@ -195,7 +195,8 @@ __global__ __launch_bounds__(256) void batch_fft_list(const BatchProblem *__rest
     const FftArgs &a = P[t.x].f[pass];
     // The lists count both face tiles in the forward passes; a table filled
     // without faces simply ends here instead of reading null face pointers.
-    const int tiles = a.nbatch * ((a.lanes + W - 1) / W) + a.xlanes * ((a.nbatch + W - 1) / W);
+    const long long tiles = uniform_main_tiles(a.nbatch, a.lanes, a.in_batch_stride, a.out_batch_stride, W) +
+                            (long long)a.xlanes * ((a.nbatch + W - 1) / W);
     if (t.y >= tiles) return;
     fft_tile_uniform<L, R1, R2, W, KIND>(a, t.y, tile);
 }
@@ -419,7 +420,8 @@ int build_lists(hz_batch *b, int rt)
                 const long long lim = 1LL << 29;  // element offsets whose byte offsets fit 32 bits
                 const long long m = (W - 1) + (long long)(L - 1) * std::max(a.in_point_stride, a.out_point_stride);
                 if (m >= lim) return set_error(HZ_ERR_UNSUPPORTED, "batch: problem too large for a tile");
-                ntask = a.nbatch * ((a.lanes + W - 1) / W) + faces * ((a.nbatch + W - 1) / W);
+                ntask = (int)uniform_main_tiles(a.nbatch, a.lanes, a.in_batch_stride, a.out_batch_stride, W) +
+                        faces * ((a.nbatch + W - 1) / W);
             }
             for (int t = 0; t < ntask; ++t) v.push_back(make_int2(p, t));
         }

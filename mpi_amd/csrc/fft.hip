@@ -58,7 +58,7 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
     constexpr int Q = R1 > R2 ? R1 : R2;
     const dim3 block(W * Q);
     if (offsets_fit_32(L, W, a)) {
-        const long long tiles = (long long)a.nbatch * ((a.lanes + W - 1) / W) +
+        const long long tiles = uniform_main_tiles(a.nbatch, a.lanes, a.in_batch_stride, a.out_batch_stride, W) +
                                 (long long)a.xlanes * ((a.nbatch + W - 1) / W);
         const dim3 grid((unsigned)tiles);
         switch (kind) {

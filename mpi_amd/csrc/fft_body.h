@@ -288,22 +288,46 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
     fft_core<L, R1, R2, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
 }
 
+// Line batches whose lanes follow each other in memory on both sides (batch
+// stride = lanes: the x passes of the half spectrum, [x][m][z]) are tiled as
+// one batch of nbatch * lanes lanes, so that only the pass's last tile is
+// partial instead of the last tile of every batch.
+__host__ __device__ inline bool lanes_merge(int nbatch, int lanes, long long in_bs, long long out_bs)
+{
+    return nbatch > 1 && in_bs == lanes && out_bs == lanes;
+}
+
+// Main-field tiles of a uniform-base launch of W lanes per tile (extra-field
+// tiles come on top: xlanes * ceil(nbatch / W)).
+__host__ __device__ inline long long uniform_main_tiles(int nbatch, int lanes, long long in_bs, long long out_bs,
+                                                        int W)
+{
+    const bool m = lanes_merge(nbatch, lanes, in_bs, out_bs);
+    const long long ln = m ? (long long)nbatch * lanes : lanes;
+    return (m ? 1 : nbatch) * ((ln + W - 1) / W);
+}
+
 // Uniform-base tiles of one launch (fft.hip): tile t < nbatch * nlb is lane
-// block t % nlb of line batch t / nlb of the main field; the rest are the
-// extra fields, xlanes * ceil(nbatch / W) tiles whose W "lanes" are W
-// consecutive line batches of one extra field.
+// block t % nlb of line batch t / nlb of the main field (one merged batch
+// when lanes_merge holds); the rest are the extra fields,
+// xlanes * ceil(nbatch / W) tiles whose W "lanes" are W consecutive line
+// batches of one extra field.
 template <int L, int R1, int R2, int W, int KIND>
 __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double *tile)
 {
     const int w = threadIdx.x % W;
-    const int nlb = (a.lanes + W - 1) / W;
-    const int nmain = a.nbatch * nlb;
+    const int nb = __builtin_amdgcn_readfirstlane(a.nbatch), ln = __builtin_amdgcn_readfirstlane(a.lanes);
+    const long long in_bs = sval(a.in_batch_stride), out_bs = sval(a.out_batch_stride);
+    const bool merged = lanes_merge(nb, ln, in_bs, out_bs);
+    const int lanes = merged ? nb * ln : ln;
+    const int nlb = (lanes + W - 1) / W;
+    const int nmain = (merged ? 1 : nb) * nlb;
     Addr32 ad;
     bool live;
     if (t < nmain) {
         const int batch = t / nlb, l0 = (t % nlb) * W;
-        const long long ib = (long long)batch * a.in_batch_stride + l0;
-        const long long ob = (long long)batch * a.out_batch_stride + l0;
+        const long long ib = (long long)batch * in_bs + l0;
+        const long long ob = (long long)batch * out_bs + l0;
         ad.ire = (gcdouble *)sval(a.in_re) + ib;
         ad.iim = (gcdouble *)sval(a.in_im) + ib;
         ad.ore = (gdouble *)sval(a.out_re) + ob;
@@ -312,9 +336,9 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ob = w;
         ad.ips = (unsigned)a.in_point_stride;
         ad.ops = (unsigned)a.out_point_stride;
-        live = l0 + w < a.lanes;
+        live = l0 + w < lanes;
     } else {
-        const int nbb = (a.nbatch + W - 1) / W;
+        const int nbb = (nb + W - 1) / W;
         const int u = t - nmain, sx = u / nbb, b0 = (u % nbb) * W;
         const bool x1 = sx == 1;
         const long long ib = (long long)b0 * a.x_in_batch_stride;
@@ -327,7 +351,7 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ob = (unsigned)(w * a.x_out_batch_stride);
         ad.ips = (unsigned)a.x_in_point_stride;
         ad.ops = (unsigned)a.x_out_point_stride;
-        live = b0 + w < a.nbatch;
+        live = b0 + w < nb;
     }
     fft_core<L, R1, R2, W, KIND>(ad, live, a.tw, tile);
 }
