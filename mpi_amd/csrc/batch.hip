@@ -308,13 +308,15 @@ int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, 
         hz_plan *pl = b->plans[p];
         BatchProblem &q = b->h_probs[p];
         q = BatchProblem{};
-        q.f[0] = xy_pass_args(pl, 0, f[p], pl->spec.re, pl->spec.im, nullptr, pl->K, gl[p], gr[p], nullptr, nullptr);
+        const long long xs = pl->xs;
+        q.f[0] = xy_pass_args(pl, 0, f[p], pl->spec.re, pl->spec.im, nullptr, pl->K, gl[p], gr[p], nullptr, nullptr,
+                              xs);
         q.f[1] = xy_pass_args(pl, 1, nullptr, pl->spec.re, pl->spec.im, nullptr, pl->K, gl[p], gr[p], nullptr,
-                              nullptr);
+                              nullptr, xs);
         q.f[2] = xy_pass_args(pl, 2, nullptr, pl->spec.re, pl->spec.im, nullptr, pl->K, nullptr, nullptr, nullptr,
-                              nullptr);
+                              nullptr, xs);
         q.f[3] = xy_pass_args(pl, 3, nullptr, pl->spec.re, pl->spec.im, u[p], pl->K, nullptr, nullptr, nullptr,
-                              nullptr);
+                              nullptr, xs);
         q.d = diag_args(pl);
         q.ly = pl->ny;
         q.lx = pl->nx;

@@ -157,7 +157,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             const int i = fc / M, m = fc % M;
             const int iw = i <= a.nx / 2 ? i : a.nx - i;
             sig = a.shift[iw * M + m];
-            col = (long long)fc * K + (long long)e * (N - 1) + (plane ? a.plane_off : 0);
+            col = (long long)i * a.xs + (long long)m * K + (long long)e * (N - 1) + (plane ? a.plane_off : 0);
         }
         row_sig[row] = sig;
         row_col[row] = col;

@@ -68,6 +68,7 @@ int fft_width_choices(int L, int *w, int max);
 // Arguments of the plain-LU mode kernel (lu.hip).
 struct LuModeArgs {
     int nx, ny, M, K, N, n, ne;
+    long long xs;            // spectrum x-row stride (>= M * K)
     const double *DzzT;      // interior operator transposed: DzzT[j*n + i] = Dzz[i+1][j+1]
     const double *Dzz;       // full N x N
     const double *Dz;        // full N x N
@@ -88,6 +89,7 @@ int lu_modes_check(int nx, int n, int ne);
 // Arguments of the z-diagonalised mode kernel (diag.hip).
 struct DiagArgs {
     int nx, M, K, N, n, ne;
+    long long xs;               // spectrum x-row stride: column (i, m) starts at i * xs + m * K
     double *spec;               // real plane; imaginary plane at spec + plane_off
     long long plane_off;
     const double *bc_re, *bc_im;
@@ -119,6 +121,7 @@ struct MixedSetupArgs {
 };
 struct MixedArgs {
     int nx, M, K, N, n, ne, ldx, sweeps, modes;
+    long long xs;               // spectrum x-row stride (>= M * K)
     double *spec;               // real plane; imaginary plane at spec + plane_off
     long long plane_off;
     const double *bc_re, *bc_im, *shift;
@@ -160,8 +163,15 @@ namespace hz {
 //   1 C2C along x forward (re/im and, with gl/gr, the boundary spectra),
 //   2 C2C along x inverse (boundary spectra too when gl_out/gr_out given),
 //   3 C2R along y (re/im -> out; boundary spectra -> gl_out, gr_out).
+// xs: x-row stride of the re/im half-spectrum buffers (0 = M * lanes, the
+// unpadded layout of the public transform entry points).
 FftArgs xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, double *im, double *out,
-                     int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out);
+                     int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out,
+                     long long xs = 0);
+// x-row stride of a plan's own spectrum: M * K rounded up to 16 doubles, so
+// that every 16-lane row segment of the merged x passes starts on a
+// 128-byte line.
+inline long long spectrum_xs(int M, int K) { return ((long long)M * K + 15) / 16 * 16; }
 // Arguments of the forward diagonalised mode solve on `p`'s workspace.
 DiagArgs diag_args(const hz_plan *p);
 }  // namespace hz
@@ -171,6 +181,7 @@ struct hz_plan {
     hz_plan_desc d{};
     int device = 0;
     int nx = 0, ny = 0, N = 0, n = 0, ne = 0, K = 0, M = 0, modes = 0;
+    long long xs = 0;  // x-row stride of spec (spectrum_xs(M, K))
     double Lx = 0, Ly = 0, Lz = 0;
     std::vector<double> z, Dz, Dzz, shift;
     // device

@@ -365,7 +365,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
                     const int e = (r - 2) / CPE, cc = (r - 2) % CPE;
                     if (cc < nc) {
                         const int xi = (cc >> 1) ? xi1 : xi0;
-                        const long long col = ((long long)xi * M + jw) * K;
+                        const long long col = (long long)xi * a.xs + (long long)jw * K;
                         const double *src = (cc & 1) ? a.spec_im : a.spec_re;
                         v = src[col + (long long)e * (N - 1) + 1 + i];
                     }
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
         if (cc >= nc) continue;
         const double zl = zeta[0][CPE * e + cc], zr = zeta[1][CPE * e + cc];
         const int xi = (cc >> 1) ? xi1 : xi0;
-        const long long col = ((long long)xi * M + jw) * K + (long long)e * (N - 1);
+        const long long col = (long long)xi * a.xs + (long long)jw * K + (long long)e * (N - 1);
         double *dst = (cc & 1) ? a.spec_im : a.spec_re;
 #pragma unroll
         for (int aa = 0; aa < RB; ++aa) {

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(NPAD *(NPAD / JC)) void mixed_modes_kernel(const Mi
     auto col_off = [&](int c) -> long long {
         const int e = c / nc, cc = c % nc;
         const int xi = (cc >> 1) ? xi1 : xi0;
-        return ((long long)xi * M + jw) * K + (long long)e * (N - 1) + ((cc & 1) ? a.plane_off : 0);
+        return (long long)xi * a.xs + (long long)jw * K + (long long)e * (N - 1) + ((cc & 1) ? a.plane_off : 0);
     };
 
     // ---- load b; FP32 operator into registers ----

@@ -290,9 +290,11 @@ int setup_diag(hz_plan *p)
 
 // Arguments of the four line-transform passes of a solve (see hz_internal.h).
 FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, double *im, double *out,
-                         int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out)
+                         int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out,
+                         long long xs)
 {
     const long long side = (long long)p->nx * p->M;
+    const long long xrow = xs ? xs : (long long)p->M * lanes;  // spectrum x-row stride
     FftArgs a{};
     a.lanes = lanes;
     a.w = p->fft_w[pass];
@@ -302,7 +304,7 @@ FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *r
         a.out_im = im;
         a.in_batch_stride = (long long)p->ny * lanes;
         a.in_point_stride = lanes;
-        a.out_batch_stride = (long long)p->M * lanes;
+        a.out_batch_stride = xrow;
         a.out_point_stride = lanes;
         a.nbatch = p->nx;
         a.tw = p->d_twy;
@@ -325,9 +327,9 @@ FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *r
         a.out_re = re;
         a.out_im = im;
         a.in_batch_stride = lanes;
-        a.in_point_stride = (long long)p->M * lanes;
+        a.in_point_stride = xrow;
         a.out_batch_stride = lanes;
-        a.out_point_stride = (long long)p->M * lanes;
+        a.out_point_stride = xrow;
         a.nbatch = p->M;
         a.tw = p->d_twx;
         if ((pass == 1 && gl && gr) || (pass == 2 && gl_out && gr_out)) {
@@ -343,7 +345,7 @@ FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *r
         a.in_re = re;
         a.in_im = im;
         a.out_re = out;
-        a.in_batch_stride = (long long)p->M * lanes;
+        a.in_batch_stride = xrow;
         a.in_point_stride = lanes;
         a.out_batch_stride = (long long)p->ny * lanes;
         a.out_point_stride = lanes;
@@ -372,23 +374,25 @@ namespace {
 // with gl/gr the two face planes ride along as extra lanes of the same
 // launches and land in the boundary spectra.
 int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s,
-               const double *gl = nullptr, const double *gr = nullptr)
+               const double *gl = nullptr, const double *gr = nullptr, long long xs = 0)
 {
-    int st = launch_fft(p->ny, FFT_R2C, xy_pass_args(p, 0, in, re, im, nullptr, lanes, gl, gr, nullptr, nullptr), s);
+    int st =
+        launch_fft(p->ny, FFT_R2C, xy_pass_args(p, 0, in, re, im, nullptr, lanes, gl, gr, nullptr, nullptr, xs), s);
     if (st) return st;
-    return launch_fft(p->nx, FFT_C2C_FWD, xy_pass_args(p, 1, nullptr, re, im, nullptr, lanes, gl, gr, nullptr, nullptr), s);
+    return launch_fft(p->nx, FFT_C2C_FWD,
+                      xy_pass_args(p, 1, nullptr, re, im, nullptr, lanes, gl, gr, nullptr, nullptr, xs), s);
 }
 
 // Inverse transform; with gl_out/gr_out the boundary spectra are inverted
 // into them in the same launches.
 int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipStream_t s,
-               double *gl_out = nullptr, double *gr_out = nullptr)
+               double *gl_out = nullptr, double *gr_out = nullptr, long long xs = 0)
 {
     int st = launch_fft(p->nx, FFT_C2C_INV,
-                        xy_pass_args(p, 2, nullptr, re, im, nullptr, lanes, nullptr, nullptr, gl_out, gr_out), s);
+                        xy_pass_args(p, 2, nullptr, re, im, nullptr, lanes, nullptr, nullptr, gl_out, gr_out, xs), s);
     if (st) return st;
     return launch_fft(p->ny, FFT_C2R,
-                      xy_pass_args(p, 3, nullptr, re, im, out, lanes, nullptr, nullptr, gl_out, gr_out), s);
+                      xy_pass_args(p, 3, nullptr, re, im, out, lanes, nullptr, nullptr, gl_out, gr_out, xs), s);
 }
 
 }  // namespace
@@ -405,6 +409,7 @@ DiagArgs hz::diag_args(const hz_plan *p)
     a.ne = p->ne;
     a.spec = p->spec.re;
     a.plane_off = p->spec.im - p->spec.re;
+    a.xs = p->xs;
     a.bc_re = p->bc.re;
     a.bc_im = p->bc.im;
     a.shift = p->d_shift;
@@ -434,7 +439,7 @@ int tune_diag(hz_plan *p)
 {
     int choices[8];
     const int k = diag_tile_choices(p->NP, choices, 8);
-    HZ_HIP(hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->M * p->K));
+    HZ_HIP(hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->xs));
     HZ_HIP(hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M));
     hipStream_t s;
     hipEvent_t e0, e1;
@@ -481,7 +486,7 @@ int tune_fft(hz_plan *p)
         return set_error(HZ_ERR_ALLOC, "hipMalloc: tuning buffer");
     int st = HZ_OK;
     if (hipMemset(in, 0, sizeof(double) * phys) != hipSuccess ||
-        hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->M * p->K) != hipSuccess ||
+        hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->xs) != hipSuccess ||
         hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M) != hipSuccess)
         st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
     hipStream_t s = nullptr;
@@ -500,7 +505,7 @@ int tune_fft(hz_plan *p)
             p->fft_w[pass] = w[i];
             // the faces ride along as in a solve (passes 0 and 1)
             FftArgs a = xy_pass_args(p, pass, in, p->spec.re, p->spec.im, in, p->K, pass < 2 ? in : nullptr,
-                                     pass < 2 ? in : nullptr, nullptr, nullptr);
+                                     pass < 2 ? in : nullptr, nullptr, nullptr, p->xs);
             if ((st = launch_fft(L, kinds[pass], a, s))) break;
             (void)hipEventRecord(e0, s);
             for (int r = 0; r < 3 && !st; ++r) st = launch_fft(L, kinds[pass], a, s);
@@ -603,7 +608,8 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     std::vector<double> DzzT((size_t)p->n * p->n);
     for (int i = 0; i < p->n; ++i)
         for (int j = 0; j < p->n; ++j) DzzT[(size_t)j * p->n + i] = p->Dzz[(size_t)(i + 1) * p->N + (j + 1)];
-    const size_t specN = (size_t)p->nx * p->M * p->K, bcN = (size_t)2 * p->nx * p->M;
+    p->xs = spectrum_xs(p->M, p->K);
+    const size_t specN = (size_t)p->nx * p->xs, bcN = (size_t)2 * p->nx * p->M;
     if ((st = upload(&p->d_Dz, p->Dz)) || (st = upload(&p->d_Dzz, p->Dzz)) ||
         (st = upload(&p->d_DzzT, DzzT)) || (st = dmalloc(&p->d_shift, p->modes)) ||
         (st = upload(&p->d_twx, twiddles(p->nx))) || (st = upload(&p->d_twy, twiddles(p->ny))) ||
@@ -688,7 +694,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         return set_error(HZ_ERR_ARG, "null face data");
     }
     StageRecorder rec(p, s);
-    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr);
+    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr, p->xs);
     rec.mark("fft_forward");
     if (st) return st;
     if (p->d.form == HZ_FORM_PLAIN_LU || op_diag || op_full) {
@@ -708,6 +714,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.op_full = op_full;
         a.spec_re = p->spec.re;
         a.spec_im = p->spec.im;
+        a.xs = p->xs;
         a.bc_re = p->bc.re;
         a.bc_im = p->bc.im;
         a.alpha_l = p->d.alpha_l;
@@ -735,6 +742,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.modes = p->modes;
         a.spec = p->spec.re;
         a.plane_off = p->spec.im - p->spec.re;
+        a.xs = p->xs;
         a.bc_re = p->bc.re;
         a.bc_im = p->bc.im;
         a.shift = p->d_shift;
@@ -757,7 +765,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         st = set_error(HZ_ERR_UNSUPPORTED, "form not built yet");
     }
     if (st) return st;
-    st = inverse_xy(p, p->spec.re, p->spec.im, u, p->K, s);
+    st = inverse_xy(p, p->spec.re, p->spec.im, u, p->K, s, nullptr, nullptr, p->xs);
     rec.mark("fft_inverse");
     return st;
 }
@@ -843,7 +851,7 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
         if ((st = setup_diag(p)) || (st = compute_homogeneous(p))) return st;
     }
     StageRecorder rec(p, s);
-    if ((st = forward_xy(p, ubar, p->spec.re, p->spec.im, p->K, s))) return st;
+    if ((st = forward_xy(p, ubar, p->spec.re, p->spec.im, p->K, s, nullptr, nullptr, p->xs))) return st;
     rec.mark("fft_forward");
     DiagArgs a = diag_args(p);
     a.adjoint = 1;
@@ -853,7 +861,7 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
     a.gbar_im = p->bc.im;
     if ((st = launch_diag_modes(a, s))) return st;
     rec.mark("diag_adjoint");
-    if ((st = inverse_xy(p, p->spec.re, p->spec.im, fbar, p->K, s, glbar, grbar))) return st;
+    if ((st = inverse_xy(p, p->spec.re, p->spec.im, fbar, p->K, s, glbar, grbar, p->xs))) return st;
     rec.mark("fft_inverse");
     if (lambda_bar) {
         if (!p->d_partial && (st = dmalloc(&p->d_partial, 1025))) return st;
