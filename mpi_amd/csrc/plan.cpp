@@ -476,7 +476,8 @@ int tune_diag(hz_plan *p)
 }
 
 // Plan-time autotuning of the transform passes' lane width: each choice of
-// each pass is timed (one warm-up, three timed launches) on scratch buffers
+// each pass is timed (a warm-up, then four rounds over the choices of three
+// timed launches each, the best round kept) on scratch buffers
 // of the plan's shape and the fastest kept.
 int tune_fft(hz_plan *p)
 {
@@ -499,25 +500,32 @@ int tune_fft(hz_plan *p)
         const int L = (pass == 0 || pass == 3) ? p->ny : p->nx;
         int w[4];
         const int k = fft_width_choices(L, w, 4);
-        int bestw = 0;
-        double best = 1e300;
-        for (int i = 0; i < k && !st; ++i) {
-            p->fft_w[pass] = w[i];
-            // the faces ride along as in a solve (passes 0 and 1)
-            FftArgs a = xy_pass_args(p, pass, in, p->spec.re, p->spec.im, in, p->K, pass < 2 ? in : nullptr,
-                                     pass < 2 ? in : nullptr, nullptr, nullptr, p->xs);
-            if ((st = launch_fft(L, kinds[pass], a, s))) break;
-            (void)hipEventRecord(e0, s);
-            for (int r = 0; r < 3 && !st; ++r) st = launch_fft(L, kinds[pass], a, s);
-            (void)hipEventRecord(e1, s);
-            if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) {
-                best = ms;
-                bestw = w[i];
+        // rounds over the choices in turn (so that clock drift hits all of
+        // them alike), each choice's best round kept
+        double ms_of[4] = {1e300, 1e300, 1e300, 1e300};
+        for (int round = 0; round < 4 && !st; ++round) {
+            for (int i = 0; i < k && !st; ++i) {
+                p->fft_w[pass] = w[i];
+                // the faces ride along as in a solve (passes 0 and 1)
+                FftArgs a = xy_pass_args(p, pass, in, p->spec.re, p->spec.im, in, p->K, pass < 2 ? in : nullptr,
+                                         pass < 2 ? in : nullptr, nullptr, nullptr, p->xs);
+                if (round == 0 && (st = launch_fft(L, kinds[pass], a, s))) break;  // warm-up
+                (void)hipEventRecord(e0, s);
+                for (int r = 0; r < 3 && !st; ++r) st = launch_fft(L, kinds[pass], a, s);
+                (void)hipEventRecord(e1, s);
+                if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms_of[i] = std::min(ms_of[i], (double)ms);
             }
         }
+        int bestw = 0;
+        double best = 1e300;
+        for (int i = 0; i < k; ++i)
+            if (ms_of[i] < best) {
+                best = ms_of[i];
+                bestw = w[i];
+            }
         p->fft_w[pass] = bestw;
         p->fft_w_ms[pass] = k ? best / 3.0 : 0.0;
     }
