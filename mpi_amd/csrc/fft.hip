@@ -3,7 +3,8 @@
 // The fields are [a][b][K] with K (z) contiguous, so a transform along a or
 // b is a batch of K independent lines whose same-index points sit in one
 // contiguous run of K doubles.  A work-group takes W consecutive z lanes of
-// one line-batch and the whole transform length L:
+// one line-batch (or, where consecutive line batches are contiguous, W
+// consecutive lanes of their concatenation) and the whole transform length L:
 //   stage 1: each thread loads R1 points x[n2 + R2*n1] of its lane with
 //            coalesced W-wide loads, does an R1-point DFT in registers and
 //            multiplies by the twiddle W_L^(n2*k1);
