@@ -201,6 +201,16 @@ __global__ __launch_bounds__(256) void batch_fft_list(const BatchProblem *__rest
     fft_tile_uniform<L, R1, R2, W, KIND>(a, t.y, tile);
 }
 
+// Length-1 transform passes (2-D and 1-D problems): 256 (batch, lane)
+// copies per list entry.
+template <int KIND>
+__global__ __launch_bounds__(256) void batch_unit_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list,
+                                                       int pass)
+{
+    const int2 t = list[blockIdx.x];
+    unit_line_element<KIND>(P[t.x].f[pass], (long long)t.y * 256 + threadIdx.x);
+}
+
 template <int NJ, int RT, bool GENERAL>
 __global__ __launch_bounds__(256) void batch_diag_list(const BatchProblem *__restrict__ P, const int2 *__restrict__ list)
 {
@@ -337,6 +347,7 @@ template <int KIND>
 void launch_fft_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int L, int pass)
 {
     switch (L) {
+    case 1: hipLaunchKernelGGL((batch_unit_list<KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
     case 16: hipLaunchKernelGGL((batch_fft_list<16, 4, 4, 64, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
     case 32: hipLaunchKernelGGL((batch_fft_list<32, 8, 4, 32, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
     case 64: hipLaunchKernelGGL((batch_fft_list<64, 8, 8, 32, KIND>), dim3(count), dim3(256), 0, s, P, list, pass); break;
@@ -382,6 +393,7 @@ int enqueue_t(hz_batch *b, const Variant &v, hipStream_t s)
         const auto &g = b->groups[r];
         for (size_t i = 0; i + 1 < g.size(); ++i) {
             const int st = g[i][0], kind = g[i][1], first = g[i][2], count = g[i + 1][2] - first;
+            if (count == 0) continue;  // e.g. only in-place length-1 passes (identities) of this kind
             const int2 *list = b->d_list[r] + first;
             const int pass = st < 2 ? st : st - 1;
             switch (st) {
@@ -419,6 +431,12 @@ int build_lists(hz_batch *b, int rt)
                 const bool has_faces = !b->plans[p]->d.periodic_z;
                 const int L = task_kind(b->h_probs[p], st), W = batch_lanes(L);
                 const int faces = st < 2 && has_faces ? 2 : 0;
+                if (L == 1) {  // copies, 256 per task; the in-place x passes are identities
+                    const long long total = (long long)(a.lanes + faces) * a.nbatch;
+                    ntask = (st == 1 || st == 3) ? 0 : (int)((total + 255) / 256);
+                    for (int t = 0; t < ntask; ++t) v.push_back(make_int2(p, t));
+                    continue;
+                }
                 const long long lim = 1LL << 29;  // element offsets whose byte offsets fit 32 bits
                 const long long m = (W - 1) + (long long)(L - 1) * std::max(a.in_point_stride, a.out_point_stride);
                 if (m >= lim) return set_error(HZ_ERR_UNSUPPORTED, "batch: problem too large for a tile");
@@ -496,8 +514,8 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         if (pl->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG-form plans only");
         if (pl->device != dev) return set_error(HZ_ERR_ARG, "batch: plans on another device");
         if (pl->n > 128) return set_error(HZ_ERR_UNSUPPORTED, "batch: N-2 must be <= 128");
-        if (pl->nx < 16 || pl->nx > 256 || pl->ny < 16 || pl->ny > 256)
-            return set_error(HZ_ERR_UNSUPPORTED, "batch: nx, ny must be in [16, 256]");
+        if ((pl->nx != 1 && (pl->nx < 16 || pl->nx > 256)) || (pl->ny != 1 && (pl->ny < 16 || pl->ny > 256)))
+            return set_error(HZ_ERR_UNSUPPORTED, "batch: nx, ny must be 1 or in [16, 256]");
         if (2 * pl->ne > 32) return set_error(HZ_ERR_UNSUPPORTED, "batch: at most 16 elements");
         const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
         shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +

@@ -85,37 +85,12 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
     return HZ_OK;
 }
 
-// Length-1 lines (an axis of one point: the 2-D and 1-D problems): the
-// transform of one point is the point itself (1/L = 1 on the inverse), so
-// each lane is copied; R2C writes a zero imaginary part, C2R keeps the real
-// part.  One thread per (batch, lane), lanes fastest.
+// Length-1 lines (the 2-D and 1-D problems): one thread per (batch, lane),
+// unit_line_element in fft_body.h.
 template <int KIND>
 __global__ __launch_bounds__(256) void fft_unit_lines(const FftArgs a)
 {
-    const int nl = a.lanes + a.xlanes;
-    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (long long)nl * a.nbatch) return;
-    const int b = (int)(t / nl), l = (int)(t % nl);
-    const double *ire, *iim = nullptr;
-    double *ore, *oim = nullptr;
-    if (l < a.lanes) {
-        const long long io = (long long)b * a.in_batch_stride + l, oo = (long long)b * a.out_batch_stride + l;
-        ire = a.in_re + io;
-        if (KIND != FFT_R2C) iim = a.in_im + io;
-        ore = a.out_re + oo;
-        if (KIND != FFT_C2R) oim = a.out_im + oo;
-    } else {
-        const int sx = l - a.lanes;
-        const long long io = (long long)b * a.x_in_batch_stride, oo = (long long)b * a.x_out_batch_stride;
-        ire = a.xin_re[sx] + io;
-        if (KIND != FFT_R2C) iim = a.xin_im[sx] + io;
-        ore = a.xout_re[sx] + oo;
-        if (KIND != FFT_C2R) oim = a.xout_im[sx] + oo;
-    }
-    const double re = *ire;
-    const double im = (KIND == FFT_C2C_FWD || KIND == FFT_C2C_INV) ? *iim : 0.0;
-    *ore = re;
-    if (KIND != FFT_C2R) *oim = im;
+    unit_line_element<KIND>(a, (long long)blockIdx.x * 256 + threadIdx.x);
 }
 
 int launch_unit(FftKind kind, const FftArgs &a, hipStream_t s)

@@ -356,5 +356,38 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
     fft_core<L, R1, R2, W, KIND>(ad, live, a.tw, tile);
 }
 
+// Element t of a length-1 line pass (an axis of one point: the 2-D and 1-D
+// problems): the transform of one point is the point itself (1/L = 1 on the
+// inverse), so each lane is copied; R2C writes a zero imaginary part, C2R
+// keeps the real part.  t runs over (batch, lane), lanes (main field, then
+// the extra fields) fastest; t past the end does nothing.
+template <int KIND>
+__device__ __forceinline__ void unit_line_element(const FftArgs &a, long long t)
+{
+    const int nl = a.lanes + a.xlanes;
+    if (t >= (long long)nl * a.nbatch) return;
+    const int b = (int)(t / nl), l = (int)(t % nl);
+    const double *ire, *iim = nullptr;
+    double *ore, *oim = nullptr;
+    if (l < a.lanes) {
+        const long long io = (long long)b * a.in_batch_stride + l, oo = (long long)b * a.out_batch_stride + l;
+        ire = a.in_re + io;
+        if (KIND != FFT_R2C) iim = a.in_im + io;
+        ore = a.out_re + oo;
+        if (KIND != FFT_C2R) oim = a.out_im + oo;
+    } else {
+        const int sx = l - a.lanes;
+        const long long io = (long long)b * a.x_in_batch_stride, oo = (long long)b * a.x_out_batch_stride;
+        ire = a.xin_re[sx] + io;
+        if (KIND != FFT_R2C) iim = a.xin_im[sx] + io;
+        ore = a.xout_re[sx] + oo;
+        if (KIND != FFT_C2R) oim = a.xout_im[sx] + oo;
+    }
+    const double re = *ire;
+    const double im = (KIND == FFT_C2C_FWD || KIND == FFT_C2C_INV) ? *iim : 0.0;
+    *ore = re;
+    if (KIND != FFT_C2R) *oim = im;
+}
+
 }  // namespace
 }  // namespace hz

@@ -222,3 +222,32 @@ def test_batch_with_periodic_problems(cuda):
     with pytest.raises(HzError):  # face data given for a periodic problem
         b.solve(fs, gls[:3] + [fs[3][:, :, 0].contiguous()] + gls[4:], grs)
     b.close()
+
+
+def test_batch_with_two_and_one_dimensional_problems(cuda):
+    """Plans with a unit Fourier axis (the 2-D solvers: nx = 1; the 1-D ones:
+    nx = ny = 1), Robin, multi-element and periodic, batched with a 3-D
+    problem: every problem equals its single-plan solve."""
+    from mpi_amd.solver import BatchSolver, HelmholtzPlan
+    specs = [  # (nx, ny, N, ne, lam, robin, periodic)
+        (1, 32, 33, 1, 2.0, False, False), (1, 64, 17, 2, 0.5, True, False), (1, 16, 65, 1, 3.0, False, True),
+        (1, 1, 129, 1, 1.5, False, False), (1, 1, 33, 3, 4.0, True, False), (1, 1, 17, 2, 2.5, False, True),
+        (32, 16, 33, 1, 1.0, False, False)]
+    plans, fs, gls, grs = [], [], [], []
+    for k, (nx, ny, N, ne, lam, robin, periodic) in enumerate(specs):
+        al, bl, ar, br = (0.7, 1.3, 0.4, 2.0) if robin else (0.0, 1.0, 0.0, 1.0)
+        p = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam,
+                          alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form="diag", periodic_z=periodic)
+        rng = np.random.default_rng(500 + k)
+        plans.append(p)
+        fs.append(dev(rng.standard_normal((nx, ny, p.nz))))
+        gls.append(None if periodic else dev(rng.standard_normal((nx, ny))))
+        grs.append(None if periodic else dev(rng.standard_normal((nx, ny))))
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    b = BatchSolver(plans)
+    for rep in range(2):  # the tuning solve, then a graph replay
+        us = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(us, want)):
+            assert rel(u.cpu().numpy(), w) <= 1e-13, f"solve {rep}, problem {k} {specs[k]}"
+    b.close()
