@@ -266,43 +266,57 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     }
     if constexpr (ADJ) {
         __syncthreads();
-        // transposed sweep (one element): zbar -> gbar, P0bar, PNbar (Robin
-        // faces) or the transposed cyclic closure (periodic z)
+        // transposed interface system per real column: zbar -> gbar and the
+        // adjoints P0bar, PNbar of each element's end-point derivatives
+        // (Robin faces), or the transposed cyclic closure (periodic z, one
+        // element).  Rows that belong to no column stay zero.
         for (int row = tid; row < RT; row += 256) {
-            const int fl = row >> 1, plane = row & 1;
-            const int fc = fc0 + fl;
-            double P0b = 0.0, PNb = 0.0;
-            if (fl < fc_per_tile && fc < nfc) {
-                const int i = fc / M, m = fc % M;
-                const int iw = i <= a.nx / 2 ? i : a.nx - i;
-                const int mode = iw * M + m;
-                const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
-                const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
-                if (a.periodic) {
-                    // u_0 = u_{N-1} = z = (dN - d0) / bp (periodic_sweep, one
-                    // element): zbar collects both end values and the load
-                    // through ul + ur; d0bar = -zbar / bp, dNbar = zbar / bp
-                    const double bp = (L0 + R0) - (LN + RN);
-                    const double zb = (row_u0[row] + row_uN[row]) + (row_d0[row] + row_dN[row]);
-                    P0b = -zb / bp;
-                    PNb = zb / bp;
-                } else {
-                    const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
-                    const double lb = a.alpha_r * LN;
-                    const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
-                    const double zlb = row_u0[row] + row_d0[row];
-                    const double zrb = row_uN[row] + row_dN[row] - (c / b) * zlb;
-                    const double rhsrb = zrb / br;
-                    const double rhsb = zlb / b - (lb / b) * rhsrb;
-                    P0b = a.alpha_l * rhsb;
-                    PNb = -a.alpha_r * rhsrb;
-                    double *gre = plane ? a.gbar_im : a.gbar_re;
-                    gre[fc] = rhsb;
-                    gre[(long long)nfc + fc] = rhsrb;
-                }
+            const int sc = row / E;
+            if (sc >= 2 * fc_per_tile || fc0 + (sc >> 1) >= nfc) {
+                row_zl[row] = 0.0;
+                row_zr[row] = 0.0;
             }
-            row_zl[row] = P0b;  // reused: P0bar, PNbar per row
-            row_zr[row] = PNb;
+        }
+        for (int sc = tid; sc < 2 * fc_per_tile; sc += 256) {
+            const int fl = sc >> 1, plane = sc & 1;
+            const int fc = fc0 + fl;
+            if (fc >= nfc) continue;
+            const int i = fc / M, m = fc % M;
+            const int iw = i <= a.nx / 2 ? i : a.nx - i;
+            const int mode = iw * M + m;
+            const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
+            const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
+            const int r0w = sc * E;  // the column's element rows
+            if (a.periodic) {
+                // u_0 = u_{N-1} = z = (dN - d0) / bp (periodic_sweep, one
+                // element): zbar collects both end values and the load
+                // through ul + ur; d0bar = -zbar / bp, dNbar = zbar / bp
+                const double bp = (L0 + R0) - (LN + RN);
+                const double zb = (row_u0[r0w] + row_uN[r0w]) + (row_d0[r0w] + row_dN[r0w]);
+                row_zl[r0w] = -zb / bp;
+                row_zr[r0w] = zb / bp;
+                continue;
+            }
+            double *gre = plane ? a.gbar_im : a.gbar_re;
+            if (E == 1) {
+                const double b = -a.alpha_l * L0 + a.beta_l, c = -a.alpha_l * R0;
+                const double lb = a.alpha_r * LN;
+                const double br = a.alpha_r * RN + a.beta_r - (c / b) * lb;
+                const double zlb = row_u0[r0w] + row_d0[r0w];
+                const double zrb = row_uN[r0w] + row_dN[r0w] - (c / b) * zlb;
+                const double rhsrb = zrb / br;
+                const double rhsb = zlb / b - (lb / b) * rhsrb;
+                row_zl[r0w] = a.alpha_l * rhsb;  // reused: P0bar, PNbar per row
+                row_zr[r0w] = -a.alpha_r * rhsrb;
+                gre[fc] = rhsb;
+                gre[(long long)nfc + fc] = rhsrb;
+                continue;
+            }
+            double glb, grb;
+            robin_sweep_adjoint(E, L0, LN, R0, RN, a.alpha_l, a.beta_l, a.alpha_r, a.beta_r, row_u0 + r0w,
+                                row_uN + r0w, row_d0 + r0w, row_dN + r0w, row_zl + r0w, row_zr + r0w, &glb, &grb);
+            gre[fc] = glb;
+            gre[(long long)nfc + fc] = grb;
         }
     } else if (GENERAL) {
         __syncthreads();

@@ -107,5 +107,54 @@ __device__ __forceinline__ void robin_sweep(int E, double d0ul, double dNul, dou
     }
 }
 
+// Transpose of robin_sweep's interface system T zeta = r (unknowns zeta_0
+// .. zeta_E: the face and interface values; rows: left Robin, E - 1 slope
+// matching conditions, right Robin), for the adjoint of a column of E > 1
+// elements.  Inputs per element e: u0[e], uN[e] the adjoints at its first
+// and last node (a shared interface node is read from u0 of the element to
+// its right), w0y[e], wNy[e] the loads (w0 / (mu - sigma)) . y_e and
+// (wN / (mu - sigma)) . y_e through ul, ur.  zetabar_i collects them,
+// lambda = T^-T zetabar, and since r_0 = g_l + alpha_l d0_0,
+// r_e = d0_e - dN_(e-1), r_E = g_r - alpha_r dN_(E-1):
+//   p0b[e] = d0bar_e, pNb[e] = dNbar_e, *glb = lambda_0, *grb = lambda_E.
+// p0b, pNb carry the eliminated diagonal and right-hand side until the back
+// substitution overwrites them.
+__device__ __forceinline__ void robin_sweep_adjoint(int E, double d0ul, double dNul, double d0ur, double dNur,
+                                                    double al, double bl, double ar, double br, const double *u0,
+                                                    const double *uN, const double *w0y, const double *wNy,
+                                                    double *p0b, double *pNb, double *glb, double *grb)
+{
+    // T^T: diagonal d_i, sub-diagonal a_i (coefficient of lambda_(i-1)),
+    // super-diagonal s_i (coefficient of lambda_(i+1))
+    auto diag = [&](int i) { return i == 0 ? -al * d0ul + bl : i == E ? ar * dNur + br : dNur - d0ul; };
+    auto sub = [&](int i) { return i == 1 ? -al * d0ur : -d0ur; };
+    auto sup = [&](int i) { return i == E - 1 ? ar * dNul : dNul; };
+    auto zbar = [&](int i) {
+        return i == 0 ? u0[0] + w0y[0] : i == E ? uN[E - 1] + wNy[E - 1] : u0[i] + w0y[i] + wNy[i - 1];
+    };
+    double dp = diag(0), yp = zbar(0);
+    p0b[0] = dp;
+    pNb[0] = yp;
+    for (int i = 1; i <= E; ++i) {
+        const double m = sub(i) / dp;
+        dp = diag(i) - m * sup(i - 1);
+        yp = zbar(i) - m * yp;
+        if (i < E) {
+            p0b[i] = dp;
+            pNb[i] = yp;
+        }
+    }
+    const double lamE = yp / dp;
+    double next = lamE;
+    for (int i = E - 1; i >= 0; --i) {
+        const double lam = (pNb[i] - sup(i) * next) / p0b[i];
+        pNb[i] = i == E - 1 ? -ar * lamE : -next;
+        p0b[i] = i == 0 ? al * lam : lam;
+        next = lam;
+    }
+    *glb = next;
+    *grb = lamE;
+}
+
 }  // namespace
 }  // namespace hz

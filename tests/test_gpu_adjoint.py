@@ -28,18 +28,18 @@ DIRICHLET = (0.0, 1.0, 0.0, 1.0)
 NEUMANN = (1.0, 0.0, 1.0, 0.0)
 
 
-def make_plan(nx, ny, N, bc, lam, form, yr=3.0):
+def make_plan(nx, ny, N, bc, lam, form, yr=3.0, ne=1):
     from mpi_amd.solver import HelmholtzPlan
     al, bl, ar, br = bc
-    return HelmholtzPlan(nx, ny, N, 1, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2, lam=lam,
+    return HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2, lam=lam,
                          alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form=form)
 
 
-def oracle_matrix(nx, ny, N, bc, lam, yr=3.0):
+def oracle_matrix(nx, ny, N, bc, lam, yr=3.0, ne=1):
     """Dense solve map of the oracle: columns = responses to unit inputs
     (f nodes, then g_l, then g_r)."""
     al, bl, ar, br = bc
-    desc = po.make_desc(nx, ny, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2,
+    desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=yr, zl=0, zr=2,
                         alphal=al, betal=bl, alphar=ar, betar=br, lam=lam)
     K = po.total_z_points(desc)
     nf, ng = nx * ny * K, nx * ny
@@ -135,13 +135,63 @@ def test_autograd_gradcheck(cuda):
     plan.close()
 
 
+@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("bc", [DIRICHLET, ROBIN, NEUMANN], ids=["dirichlet", "robin", "neumann"])
+@pytest.mark.parametrize("ne", [2, 3])
+def test_multi_element_adjoint_equals_oracle_transpose(cuda, form, bc, ne):
+    """Several elements per column: the transposed interface system (faces
+    and slope-matching rows) against the oracle's solve map, column by
+    column; shared interface nodes and every element end get fbar = 0."""
+    nx, ny, N, lam = 4, 4, 7, 2.5
+    S, K = oracle_matrix(nx, ny, N, bc, lam, ne=ne)
+    ubar = np.random.default_rng(13 + ne).standard_normal((nx, ny, K))
+    want = S.T @ ubar.ravel()
+    plan = make_plan(nx, ny, N, bc, lam, form, ne=ne)
+    fbar, glbar, grbar, _ = plan.adjoint(dev(ubar))
+    got = np.concatenate([fbar.cpu().numpy().ravel(), glbar.cpu().numpy().ravel(), grbar.cpu().numpy().ravel()])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    fb = fbar.cpu().numpy()
+    assert np.all(fb[:, :, ::N - 1] == 0.0)
+    plan.close()
+
+
+def test_multi_element_dot_product_and_lambda(cuda):
+    """Four elements at 64 x 64: the dot-product identity and dL/dlambda
+    against central differences."""
+    import torch
+    nx, N, ne, lam = 64, 33, 4, 6.0
+    plan = make_plan(nx, nx, N, ROBIN, lam, "diag", yr=2 * math.pi, ne=ne)
+    g = torch.Generator(device="cuda").manual_seed(17)
+    K = plan.nz
+    f = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    f[:, :, ::N - 1] = 0.0  # the solve ignores f on element end nodes
+    gl = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    gr = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    ubar = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    u = plan.solve(f, gl, gr)
+    fbar, glbar, grbar, lam_bar = plan.adjoint(ubar, u)
+    lhs = float(torch.sum(u * ubar))
+    rhs = float(torch.sum(f * fbar) + torch.sum(gl * glbar) + torch.sum(gr * grbar))
+    assert abs(lhs - rhs) <= 1e-12 * float(torch.sum(torch.abs(u * ubar)))
+
+    def L(lv):
+        plan.set_lambda(lv)
+        return float(torch.sum(ubar * plan.solve(f, gl, gr)))
+
+    h = 1e-4 * lam
+    fd = (L(lam + h) - L(lam - h)) / (2 * h)
+    plan.set_lambda(lam)
+    assert abs(lam_bar - fd) <= 1e-6 * abs(fd)
+    plan.close()
+
+
 def test_adjoint_rejects_unsupported(cuda):
     import torch
     from mpi_amd._lib import HzError
     from mpi_amd.solver import HelmholtzPlan
-    plan = HelmholtzPlan(8, 8, 9, 2, form="diag")
+    plan = HelmholtzPlan(8, 8, 9, 2, form="diag", periodic_z=True)
     ubar = torch.zeros((8, 8, plan.nz), dtype=torch.float64, device="cuda")
-    with pytest.raises(HzError):
+    with pytest.raises(HzError):  # periodic z: one element only
         plan.adjoint(ubar)
     plan.close()
 
