@@ -140,8 +140,8 @@ int hz_solve_status(hz_plan *plan, hz_stream_t stream);
  *   glbar = dL/dg_l, grbar = dL/dg_r   [nx][ny]
  * and, if u != NULL (the forward solution) and lambda_bar != NULL (host),
  *   *lambda_bar = dL/dlambda = sum over nodes of fbar * u.
- * DIAG and PLAIN_LU forms, any number of elements; with periodic z one
- * element, no face data, and glbar, grbar must be NULL. */
+ * DIAG and PLAIN_LU forms, any number of elements; with periodic z there
+ * is no face data and glbar, grbar must be NULL. */
 int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
                double *glbar, double *grbar, double *lambda_bar, hz_stream_t stream);
 

@@ -22,7 +22,6 @@ int launch_t(const DiagArgs &a, hipStream_t s)
     const int blocks = (nfc + fc_per_tile - 1) / fc_per_tile;
     const bool general = a.ne > 1 || a.alpha_l != 0.0 || a.alpha_r != 0.0 || a.periodic;
     if (a.adjoint) {
-        if (a.periodic && a.ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "adjoint: periodic z, one element only");
         hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true>), dim3(blocks), dim3(256), 0, s, a);
     } else if (general) {
         hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true, false>), dim3(blocks), dim3(256), 0, s, a);

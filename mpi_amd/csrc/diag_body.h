@@ -287,6 +287,11 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             const double L0 = a.hom[4 * mode + 0], LN = a.hom[4 * mode + 1];
             const double R0 = a.hom[4 * mode + 2], RN = a.hom[4 * mode + 3];
             const int r0w = sc * E;  // the column's element rows
+            if (a.periodic && E > 1) {
+                periodic_sweep_adjoint(E, L0, LN, R0, RN, row_u0 + r0w, row_uN + r0w, row_d0 + r0w, row_dN + r0w,
+                                       row_zl + r0w, row_zr + r0w);
+                continue;
+            }
             if (a.periodic) {
                 // u_0 = u_{N-1} = z = (dN - d0) / bp (periodic_sweep, one
                 // element): zbar collects both end values and the load

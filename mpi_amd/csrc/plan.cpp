@@ -851,7 +851,6 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
     if (!p || !ubar || !fbar) return set_error(HZ_ERR_ARG, "null argument");
     if (p->d.periodic_z ? (glbar || grbar) : (!glbar || !grbar))
         return set_error(HZ_ERR_ARG, "face gradients: required without periodic z, NULL with it");
-    if (p->d.periodic_z && p->ne != 1) return set_error(HZ_ERR_UNSUPPORTED, "hz_adjoint: periodic z, one element only");
     if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
     const hipStream_t s = (hipStream_t)stream;
     int st;

@@ -156,5 +156,60 @@ __device__ __forceinline__ void robin_sweep_adjoint(int E, double d0ul, double d
     *grb = lamE;
 }
 
+// Transpose of periodic_sweep's cyclic interface system, E > 1 elements.
+// The interface values zeta_0 .. zeta_(E-1) (zeta_0 at both ends of the
+// column) solve the circulant tridiagonal system
+//   zeta_(e-1) dNul + zeta_e (dNur - d0ul) - zeta_(e+1) d0ur = d0_e - dN_(e-1)
+// (indices mod E: slope matching at every interface).  Inputs per element
+// as in robin_sweep_adjoint (u0, uN, w0y, wNy; the column's first and last
+// nodes both carry zeta_0); lambda = C^-T zetabar is solved directly (E = 2)
+// or with the Sherman-Morrison correction of a tridiagonal solve, and
+//   p0b[e] = d0bar_e = lambda_e,  pNb[e] = dNbar_e = -lambda_((e+1) mod E).
+// u0 and pNb serve as scratch once their inputs are consumed.
+__device__ __forceinline__ void periodic_sweep_adjoint(int E, double d0ul, double dNul, double d0ur, double dNur,
+                                                       double *u0, const double *uN, const double *w0y,
+                                                       const double *wNy, double *p0b, double *pNb)
+{
+    for (int e = 0; e < E; ++e)  // zetabar
+        pNb[e] = e == 0 ? (u0[0] + uN[E - 1]) + (w0y[0] + wNy[E - 1]) : u0[e] + (w0y[e] + wNy[e - 1]);
+    // C^T: sub-diagonal a (coefficient of lambda_(i-1)), diagonal b,
+    // super-diagonal c (coefficient of lambda_(i+1)), wrapping around
+    const double a = -d0ur, b = dNur - d0ul, c = dNul;
+    if (E == 2) {
+        const double o = a + c, det = b * b - o * o;
+        const double l0 = (b * pNb[0] - o * pNb[1]) / det, l1 = (b * pNb[1] - o * pNb[0]) / det;
+        p0b[0] = l0;
+        p0b[1] = l1;
+        pNb[0] = -l1;
+        pNb[1] = -l0;
+        return;
+    }
+    // tridiagonal part with the corners folded into the first and last
+    // diagonal entries, two right-hand sides: zetabar (y) and u = (g, 0, .., 0, c) (z)
+    const double g = -b, beta = a;  // corners: row 0 holds a * lambda_(E-1), row E-1 holds c * lambda_0
+    double cp = 0.0, yp = 0.0, zp = 0.0;
+    for (int i = 0; i < E; ++i) {
+        const double bi = i == 0 ? b - g : i == E - 1 ? b - c * beta / g : b;
+        const double den = i == 0 ? bi : bi - a * cp;
+        const double ui = i == 0 ? g : i == E - 1 ? c : 0.0;
+        yp = (pNb[i] - (i ? a * yp : 0.0)) / den;
+        zp = (ui - (i ? a * zp : 0.0)) / den;
+        cp = c / den;
+        p0b[i] = cp;
+        pNb[i] = yp;
+        u0[i] = zp;
+    }
+    double ynext = pNb[E - 1], znext = u0[E - 1];
+    for (int i = E - 2; i >= 0; --i) {
+        ynext = pNb[i] - p0b[i] * ynext;
+        znext = u0[i] - p0b[i] * znext;
+        pNb[i] = ynext;
+        u0[i] = znext;
+    }
+    const double f = (pNb[0] + beta * pNb[E - 1] / g) / (1.0 + u0[0] + beta * u0[E - 1] / g);
+    for (int e = 0; e < E; ++e) p0b[e] = pNb[e] - f * u0[e];
+    for (int e = 0; e < E; ++e) pNb[e] = -p0b[(e + 1) % E];
+}
+
 }  // namespace
 }  // namespace hz

@@ -185,14 +185,19 @@ def test_multi_element_dot_product_and_lambda(cuda):
     plan.close()
 
 
-def test_adjoint_rejects_unsupported(cuda):
+def test_adjoint_rejects_bad_arguments(cuda):
+    import ctypes
     import torch
-    from mpi_amd._lib import HzError
+    from mpi_amd._lib import HzError, check, load
     from mpi_amd.solver import HelmholtzPlan
     plan = HelmholtzPlan(8, 8, 9, 2, form="diag", periodic_z=True)
     ubar = torch.zeros((8, 8, plan.nz), dtype=torch.float64, device="cuda")
-    with pytest.raises(HzError):  # periodic z: one element only
-        plan.adjoint(ubar)
+    fbar = torch.empty_like(ubar)
+    g = torch.empty((8, 8), dtype=torch.float64, device="cuda")
+    VP = ctypes.c_void_p
+    with pytest.raises(HzError):  # face gradients requested of a periodic plan
+        check(load().hz_adjoint(plan.handle, VP(ubar.data_ptr()), None, VP(fbar.data_ptr()), VP(g.data_ptr()),
+                                VP(g.data_ptr()), None, None))
     plan.close()
 
 
@@ -295,3 +300,30 @@ def test_periodic_adjoint_equals_oracle_transpose(cuda, form):
     assert torch.autograd.gradcheck(lambda a, b: helmholtz_solve(a, None, None, b, small), (fs, lt),
                                     eps=1e-6, atol=1e-7, rtol=1e-6)
     small.close()
+
+
+@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("ne", [2, 3, 4])
+def test_periodic_multi_element_adjoint_equals_oracle_transpose(cuda, form, ne):
+    """Periodic z with several elements: the transposed cyclic interface
+    system (direct for two elements, Sherman-Morrison above) against the
+    oracle's periodic solve map, column by column."""
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny, N, lam = 4, 4, 7, 2.0
+    desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam)
+    K = po.total_z_points(desc)
+    nf = nx * ny * K
+    S = np.zeros((nf, nf))
+    for c in range(nf):
+        f = np.zeros(nf)
+        f[c] = 1.0
+        S[:, c] = po.solve_periodic(desc, f).ravel()
+    ubar = np.random.default_rng(40 + ne).standard_normal((nx, ny, K))
+    want = S.T @ ubar.ravel()
+    plan = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam, form=form,
+                         periodic_z=True)
+    fbar, glbar, grbar, _ = plan.adjoint(dev(ubar))
+    assert glbar is None and grbar is None
+    got = fbar.cpu().numpy().ravel()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    plan.close()
