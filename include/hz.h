@@ -87,7 +87,7 @@ typedef struct {
     int periodic_z;          /* 1: z periodic as in Periodic3D_MPI.cpp (u and
                                 u_z continuous across z = zr -> zl); the Robin
                                 coefficients and face data are not used.
-                                PLAIN_LU and DIAG forms.                   */
+                                All three forms.                           */
 } hz_plan_desc;
 
 /* Replaces CreatePDEPlan (NonPeriodic3D_MPI.cpp:40-134): GLL nodes, scaled

@@ -133,6 +133,7 @@ struct MixedArgs {
     const double *UL, *UR, *hom;
     double alpha_l, beta_l, alpha_r, beta_r;
     double *hist;               // [(sweeps+1)][modes]
+    int periodic;               // z periodic: cyclic interface system, no face data
 };
 int launch_mixed_setup(const MixedSetupArgs &a, hipStream_t s);
 int launch_mixed_modes(const MixedArgs &a, hipStream_t s);

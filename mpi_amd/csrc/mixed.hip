@@ -27,6 +27,7 @@
 // more in occupancy than it saves, measured).  FP64 operators are streamed
 // from memory at each application (few modes).
 #include "hz_internal.h"
+#include "sweep.h"
 
 namespace hz {
 namespace {
@@ -273,7 +274,11 @@ __global__ __launch_bounds__(NPAD *(NPAD / JC)) void mixed_modes_kernel(const Mi
     __syncthreads();
 
     // ---- element-interface sweep, one thread per real column chain ----
-    if (tid < nc) {
+    if (tid < nc && a.periodic) {  // cyclic system, no face data (sweep.h)
+        const int cc = tid;
+        periodic_sweep(E, a.hom[4 * mode + 0], a.hom[4 * mode + 1], a.hom[4 * mode + 2], a.hom[4 * mode + 3],
+                       &s.dots[0][cc], &s.dots[1][cc], &s.zl[cc], &s.zr[cc], nc);
+    } else if (tid < nc) {
         const int cc = tid;
         const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
         const double *bsrc = (cc & 1) ? a.bc_im : a.bc_re;

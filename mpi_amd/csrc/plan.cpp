@@ -589,8 +589,6 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     if (d.ne < 1) return set_error(HZ_ERR_ARG, "ne must be >= 1");
     if (!(d.xr > d.xl) || !(d.yr > d.yl) || !(d.zr > d.zl)) return set_error(HZ_ERR_ARG, "empty domain");
     if (d.form < HZ_FORM_PLAIN_LU || d.form > HZ_FORM_MIXED) return set_error(HZ_ERR_ARG, "unknown form");
-    if (d.periodic_z && d.form == HZ_FORM_MIXED)
-        return set_error(HZ_ERR_UNSUPPORTED, "periodic z: PLAIN_LU and DIAG forms only");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(HZ_ERR_NO_DEVICE, "no HIP device");
 
@@ -767,6 +765,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.alpha_r = p->d.alpha_r;
         a.beta_r = p->d.beta_r;
         a.hist = p->d_hist;
+        a.periodic = p->d.periodic_z;
         st = launch_mixed_modes(a, s);
         rec.mark("mixed_modes");
     } else {

@@ -1,7 +1,8 @@
-"""Periodic-z solver (Periodic3D_MPI.cpp) on the GPU, PLAIN_LU and DIAG forms,
-against the reference's own output (golden vectors) and the oracle.
+"""Periodic-z solver (Periodic3D_MPI.cpp) on the GPU, PLAIN_LU, DIAG and MIXED
+forms, against the reference's own output (golden vectors) and the oracle.
 
-Bars: PLAIN <= 1e-12, DIAG <= 1e-10 relative max-norm error.
+Bars: PLAIN <= 1e-12, DIAG and MIXED (three refinement sweeps) <= 1e-10
+relative max-norm error.
 """
 import numpy as np
 import pytest
@@ -10,7 +11,7 @@ from golden_util import load, periodic_case_inputs, sample
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
-TOL = {"plain": 1e-12, "diag": 1e-10}
+TOL = {"plain": 1e-12, "diag": 1e-10, "mixed": 1e-10}
 
 
 def dev(a):
@@ -24,8 +25,12 @@ def plan_for(d, ne, form):
                          zl=d["zl"], zr=d["zr"], lam=d["lam"], form=form, periodic_z=True)
 
 
-@pytest.mark.parametrize("form", ["plain", "diag"])
-@pytest.mark.parametrize("name", ["shipped_np1", "random_64_N33", "random_128_N129", "random_32_N9"])
+CASES = [("shipped_np1", "plain"), ("shipped_np1", "diag")] + [
+    (name, form) for name in ("random_64_N33", "random_128_N129", "random_32_N9")
+    for form in ("plain", "diag", "mixed")]  # MIXED takes at most 4 elements: not the 25-element shipped case
+
+
+@pytest.mark.parametrize("name,form", CASES)
 def test_periodic_matches_reference(cuda, name, form):
     rec = load(f"psolve_{name}.json")
     d = rec["desc"]
@@ -68,8 +73,8 @@ def test_periodic_rejects_unsupported(cuda):
     import torch
     from mpi_amd._lib import HzError
     from mpi_amd.solver import HelmholtzPlan
-    with pytest.raises(HzError):
-        HelmholtzPlan(8, 8, 9, 1, form="mixed", periodic_z=True)
+    with pytest.raises(HzError):  # MIXED: at most 4 elements
+        HelmholtzPlan(8, 8, 9, 5, form="mixed", periodic_z=True)
     p = HelmholtzPlan(8, 8, 9, 2, form="diag", periodic_z=True)
     fbar, glbar, grbar, _ = p.adjoint(torch.zeros((8, 8, p.nz), dtype=torch.float64, device="cuda"))
     assert glbar is None and grbar is None and float(fbar.abs().max()) == 0.0
