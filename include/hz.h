@@ -236,6 +236,12 @@ int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *diag_ms, in
  * rows).  For tests and tuning studies; no counterpart in the reference. */
 int hz_plan_diag_candidates(const hz_plan *plan, int *tile_rows, int max, int *count);
 int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
+/* Periodic-z plans: the closure coefficient of the cyclic element-interface
+ * system per real spectral column of the last solve (the reference's
+ * alphaPDE, Periodic3D_MPI.cpp:698), half-spectrum order:
+ * alpha[plane][i * (ny/2 + 1) + m], plane 0 = real, 1 = imaginary part;
+ * zeros with one element (the reference computes none).  Synchronises. */
+int hz_plan_periodic_closure(const hz_plan *plan, double *alpha);
 /* Number of modes of a MIXED plan that keep an FP64 operator. */
 int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 

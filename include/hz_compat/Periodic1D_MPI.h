@@ -7,9 +7,10 @@
 // reference (Periodic1D_MPI.cpp:70-87).
 //
 // Differences a caller can observe: SolvePDE leaves rhsPDE unchanged;
-// alphaPDE (the closure coefficient of the cyclic interface system) is not
-// filled; `determinant` is not updated while SolvePDE records the operator;
-// SolverForm selects the GPU method.
+// `determinant` is not updated while SolvePDE records the operator;
+// SolverForm selects the GPU method.  alphaPDE (the closure coefficient of
+// the cyclic interface system, Periodic1D_MPI.cpp:418) is filled from the
+// GPU solve, as the reference does with more than one element.
 #pragma once
 
 #include "../hz.h"

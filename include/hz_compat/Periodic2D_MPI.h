@@ -7,9 +7,11 @@
 // in the reference (Periodic2D_MPI.cpp:91-108).
 //
 // Differences a caller can observe: SolvePDE leaves rhsPDE unchanged;
-// alphaPDE (the per-column closure coefficients of the cyclic interface
-// system) is allocated but not filled; `determinant` is not updated while
-// SolvePDE records operators; SolverForm selects the GPU method.
+// `determinant` is not updated while SolvePDE records operators; SolverForm
+// selects the GPU method.  alphaPDE (the per-column closure coefficients of
+// the cyclic interface system, Periodic2D_MPI.cpp:556) is filled from the
+// GPU solve in the reference's layout, as the reference does with more
+// than one element.
 #pragma once
 
 #include "../hz.h"

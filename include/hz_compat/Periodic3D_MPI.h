@@ -8,9 +8,10 @@
 // Differences a caller can observe:
 //  - SolvePDE leaves rhsPDE unchanged (the reference overwrites it with its
 //    spectrum);
-//  - alphaPDE (the reference's per-column closure coefficients of the cyclic
-//    interface system) is allocated but not filled: the closure is formed
-//    on the GPU inside the solve;
+//  - alphaPDE (the per-column closure coefficients of the cyclic interface
+//    system, Periodic3D_MPI.cpp:698) is filled from the GPU solve in the
+//    reference's [Nx][NY] layout (zeros with one element, where the
+//    reference computes none); its values agree to rounding, not bit for bit;
 //  - `determinant` is not updated while SolvePDE records operators;
 //  - SolverForm selects the GPU method (HZ_FORM_PLAIN_LU, the reference
 //    algorithm, by default; HZ_FORM_DIAG for the z-diagonalised form when the

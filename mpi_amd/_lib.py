@@ -78,6 +78,7 @@ SIGNATURES = {
     "hz_last_error_string": (ctypes.c_char_p, []),
     "hz_version": (ctypes.c_char_p, []),
     "hz_probe_ceilings": (ctypes.c_int, [_PD, _PD, ctypes.c_size_t]),
+    "hz_plan_periodic_closure": (ctypes.c_int, [_VP, _PD]),
 }
 
 _lib = None

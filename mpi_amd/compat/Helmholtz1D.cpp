@@ -118,6 +118,11 @@ void SolvePDE(FieldVariable U[], void (*ConstructOperator)())
     for (int ii = 0; ii < Ns; ++ii) std::memcpy(&g.h_f[(size_t)ii * (N - 1)], rhsPDE[ii], sizeof(double) * N);
 #if HZC_PERIODIC
     g.solve_recorded(Lambda, 0.0, 1.0, 0.0, 1.0);
+    {  // the closure coefficient (Periodic1D_MPI.cpp), one real column
+        std::vector<double> al;
+        g.closure_full(al);
+        alphaPDE = al[0];
+    }
 #else
     g.h_g[0] = leftBC;
     g.h_g[1] = rightBC;

@@ -178,6 +178,11 @@ void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int))
             std::memcpy(&g.h_f[(size_t)i * g.K + (size_t)jj * (N - 1)], rhsPDE[jj][i], sizeof(double) * N);
 #if HZC_PERIODIC
     g.solve_recorded(Lambda, 0.0, 1.0, 0.0, 1.0);
+    {  // the closure coefficients (Periodic2D_MPI.cpp), [NX]
+        std::vector<double> al;
+        g.closure_full(al);
+        std::memcpy(alphaPDE, al.data(), sizeof(double) * NX);
+    }
 #else
     for (int i = 0; i < Nx; ++i) {
         g.h_g[i] = leftBC[i];

@@ -198,6 +198,11 @@ void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int, const int)
                             sizeof(double) * N);
 #if HZC_PERIODIC
     g.solve_recorded(Lambda, 0.0, 1.0, 0.0, 1.0);
+    {  // the closure coefficients (Periodic3D_MPI.cpp:698), [Nx][NY]
+        std::vector<double> al;
+        g.closure_full(al);
+        for (int i = 0; i < Nx; ++i) std::memcpy(alphaPDE[i], &al[(size_t)i * NY], sizeof(double) * NY);
+    }
 #else
     const size_t ng = (size_t)Nx * Ny;
     for (int i = 0; i < Nx; ++i)

@@ -153,6 +153,22 @@ void Solver::derivative_xy(std::vector<double> &stack, int lanes, int axis, int 
     (void)hipFree(d_out);
 }
 
+void Solver::closure_full(std::vector<double> &alpha)
+{
+    const int M = ny / 2 + 1;
+    std::vector<double> half((size_t)2 * nx * M);
+    hz_check(hz_plan_periodic_closure(plan, half.data()));
+    alpha.assign((size_t)nx * 2 * ny, 0.0);
+    for (int i = 0; i < nx; ++i)
+        for (int m = 0; m < ny; ++m) {
+            const bool low = m < M;
+            const int ii = low ? i : (nx - i) % nx, mm = low ? m : ny - m;
+            const size_t h = (size_t)ii * M + mm;
+            alpha[(size_t)i * 2 * ny + 2 * m] = half[h];
+            alpha[(size_t)i * 2 * ny + 2 * m + 1] = low ? half[(size_t)nx * M + h] : -half[(size_t)nx * M + h];
+        }
+}
+
 }  // namespace hzc
 
 using namespace hzc;

@@ -55,6 +55,10 @@ struct Solver {
     // Fourier derivative along axis 0 (x) or 1 (y) of a host stack
     // [nx][ny][lanes], in place.
     void derivative_xy(std::vector<double> &stack, int lanes, int axis, int order);
+    // Periodic z: the last solve's closure coefficients in the reference's
+    // full-spectrum layout alpha[i][2 m + c] (i < nx, m < ny, c = re/im),
+    // the upper half from conjugate symmetry (re kept, im negated).
+    void closure_full(std::vector<double> &alpha);
 };
 
 }  // namespace hzc

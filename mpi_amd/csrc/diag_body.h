@@ -338,7 +338,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             const int r0w = (fl * 2 + plane) * E;
             if (a.periodic) {
                 periodic_sweep(E, d0ul, dNul, d0ur, dNur, row_d0 + r0w, row_dN + r0w, row_zl + r0w,
-                               row_zr + r0w, 1);
+                               row_zr + r0w, 1, a.closure ? a.closure + (plane ? nfc : 0) + fc : nullptr);
                 continue;
             }
             const double *bsrc = plane ? a.bc_im : a.bc_re;

@@ -69,6 +69,7 @@ int fft_width_choices(int L, int *w, int max);
 struct LuModeArgs {
     int nx, ny, M, K, N, n, ne;
     long long xs;            // spectrum x-row stride (>= M * K)
+    double *closure;         // periodic z: [re|im][nx * M] closure coefficients, or null
     const double *DzzT;      // interior operator transposed: DzzT[j*n + i] = Dzz[i+1][j+1]
     const double *Dzz;       // full N x N
     const double *Dz;        // full N x N
@@ -102,6 +103,7 @@ struct DiagArgs {
     double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
     int rt;                     // tile rows (spectral columns x planes x elements); 0 = default
     int periodic;               // z periodic: cyclic interface system, no face data
+    double *closure;            // periodic z: [re|im][nx * M] closure coefficients, or null
 };
 int diag_padded(int n);
 // Tile heights offered for padded size NP (autotuned per plan).
@@ -134,6 +136,7 @@ struct MixedArgs {
     double alpha_l, beta_l, alpha_r, beta_r;
     double *hist;               // [(sweeps+1)][modes]
     int periodic;               // z periodic: cyclic interface system, no face data
+    double *closure;            // periodic z: [re|im][nx * M] closure coefficients, or null
 };
 int launch_mixed_setup(const MixedSetupArgs &a, hipStream_t s);
 int launch_mixed_modes(const MixedArgs &a, hipStream_t s);
@@ -190,6 +193,7 @@ struct hz_plan {
     double2 *d_twx = nullptr, *d_twy = nullptr;
     hz::Spectrum spec, bc;
     int *d_status = nullptr;
+    double *d_closure = nullptr;  // periodic z: closure coefficient per real spectral column
     // z-diagonalised form
     int NP = 0;
     std::vector<double> mu, V, Vinv, w0, wN, r0, rN;

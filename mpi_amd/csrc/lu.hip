@@ -461,8 +461,10 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
     // ---- element-interface sweep, one thread per real column ----
     if (threadIdx.x < nc && a.periodic) {
         const int cc = threadIdx.x;
+        const long long fc = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
         periodic_sweep(E, dots[0][0], dots[1][0], dots[0][1], dots[1][1], &dots[0][2 + cc], &dots[1][2 + cc],
-                       &zeta[0][cc], &zeta[1][cc], CPE);
+                       &zeta[0][cc], &zeta[1][cc], CPE,
+                       a.closure ? a.closure + ((cc & 1) ? (long long)a.nx * M : 0) + fc : nullptr);
     } else if (threadIdx.x < nc) {
         const int cc = threadIdx.x;
         const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;

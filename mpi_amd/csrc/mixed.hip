@@ -276,8 +276,10 @@ __global__ __launch_bounds__(NPAD *(NPAD / JC)) void mixed_modes_kernel(const Mi
     // ---- element-interface sweep, one thread per real column chain ----
     if (tid < nc && a.periodic) {  // cyclic system, no face data (sweep.h)
         const int cc = tid;
+        const long long fc = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;
         periodic_sweep(E, a.hom[4 * mode + 0], a.hom[4 * mode + 1], a.hom[4 * mode + 2], a.hom[4 * mode + 3],
-                       &s.dots[0][cc], &s.dots[1][cc], &s.zl[cc], &s.zr[cc], nc);
+                       &s.dots[0][cc], &s.dots[1][cc], &s.zl[cc], &s.zr[cc], nc,
+                       a.closure ? a.closure + ((cc & 1) ? (long long)a.nx * M : 0) + fc : nullptr);
     } else if (tid < nc) {
         const int cc = tid;
         const long long bcol = (long long)((cc >> 1) ? xi1 : xi0) * M + jw;

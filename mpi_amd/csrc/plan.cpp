@@ -77,7 +77,7 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
     for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
-                      p->d_Vinvrow, p->d_partial, p->d_X64, p->d_UL, p->d_UR, p->d_hist})
+                      p->d_Vinvrow, p->d_partial, p->d_X64, p->d_UL, p->d_UR, p->d_hist, p->d_closure})
         (void)hipFree(q);
     (void)hipFree(p->d_X32);
     (void)hipFree(p->d_slot);
@@ -427,6 +427,7 @@ DiagArgs hz::diag_args(const hz_plan *p)
     a.beta_r = p->d.beta_r;
     a.rt = p->diag_rt;
     a.periodic = p->d.periodic_z;
+    a.closure = p->d_closure;
     return a;
 }
 
@@ -621,6 +622,7 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
         (st = upload(&p->d_twx, twiddles(p->nx))) || (st = upload(&p->d_twy, twiddles(p->ny))) ||
         (st = dmalloc(&p->spec.re, 2 * specN)) || (st = dmalloc(&p->bc.re, 2 * bcN)) ||
         (st = dmalloc(&p->d_status, p->modes)) ||
+        (d.periodic_z && (st = dmalloc(&p->d_closure, 2 * (size_t)p->nx * p->M))) ||
         ((d.form == HZ_FORM_DIAG || d.form == HZ_FORM_MIXED) && (st = setup_diag(p))) ||
         (d.form == HZ_FORM_MIXED && (st = setup_mixed(p))) || (st = compute_shifts(p))) {
         free_plan(p);
@@ -629,6 +631,7 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
+    if (p->d_closure) HZ_HIP(hipMemset(p->d_closure, 0, sizeof(double) * 2 * (size_t)p->nx * p->M));
     if ((d.form == HZ_FORM_PLAIN_LU && (st = lu_modes_check(p->nx, p->n, p->ne))) || (st = tune_fft(p)) ||
         (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
         free_plan(p);
@@ -729,6 +732,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.beta_r = p->d.beta_r;
         a.status = p->d_status;
         a.periodic = p->d.periodic_z;
+        a.closure = p->d_closure;
         st = launch_lu_modes(a, p->modes, s);
         rec.mark("lu_modes");
     } else if (p->d.form == HZ_FORM_DIAG) {
@@ -766,6 +770,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.beta_r = p->d.beta_r;
         a.hist = p->d_hist;
         a.periodic = p->d.periodic_z;
+        a.closure = p->d_closure;
         st = launch_mixed_modes(a, s);
         rec.mark("mixed_modes");
     } else {
@@ -932,6 +937,15 @@ int hz_plan_set_diag_tile(hz_plan *p, int tile_rows)
     if (!known) return set_error(HZ_ERR_ARG, "not a candidate tile shape for this plan");
     if (tile_rows < 2 * p->ne) return set_error(HZ_ERR_ARG, "the tile does not hold one column's elements");
     p->diag_rt = tile_rows;
+    return HZ_OK;
+}
+
+int hz_plan_periodic_closure(const hz_plan *p, double *alpha)
+{
+    if (!p || !alpha) return set_error(HZ_ERR_ARG, "null argument");
+    if (!p->d_closure) return set_error(HZ_ERR_UNSUPPORTED, "not a periodic-z plan");
+    HZ_HIP(hipDeviceSynchronize());
+    HZ_HIP(hipMemcpy(alpha, p->d_closure, sizeof(double) * 2 * (size_t)p->nx * p->M, hipMemcpyDeviceToHost));
     return HZ_OK;
 }
 

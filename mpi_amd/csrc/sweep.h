@@ -19,8 +19,10 @@ namespace {
 // zl, zr carry b and r0 through the forward sweep and d0, dN the backward
 // values (their inputs are dead by then); r1 is recomputed in the backward
 // pass instead of stored (the same operations, so the same values).
+// alpha_out (may be null) receives the closure coefficient (E > 1 only, as
+// the reference's alphaPDE, Periodic3D_MPI.cpp:698).
 __device__ __forceinline__ void periodic_sweep(int E, double d0ul, double dNul, double d0ur, double dNur, double *d0,
-                                               double *dN, double *zl, double *zr, int st)
+                                               double *dN, double *zl, double *zr, int st, double *alpha_out = nullptr)
 {
     if (E == 1) {
         const double b = (d0ul + d0ur) - (dNul + dNur);
@@ -61,6 +63,7 @@ __device__ __forceinline__ void periodic_sweep(int E, double d0ul, double dNul, 
         dN[e * st] = ele;
     }
     const double alpha = -(d0[(E - 1) * st] + d0[0]) / (1.0 + dN[(E - 1) * st] + dN[0]);
+    if (alpha_out) *alpha_out = alpha;
     for (int e = 0; e < E; ++e) {
         const int n1 = e + 1 < E ? e + 1 : 0;
         zl[e * st] = d0[e * st] + alpha * dN[e * st];

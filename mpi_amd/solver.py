@@ -197,6 +197,13 @@ class HelmholtzPlan:
         """Pins the DIAG mode kernel's shape to one of diag_candidates()."""
         check(self.lib.hz_plan_set_diag_tile(self.handle, int(rows)))
 
+    def periodic_closure(self):
+        """Closure coefficients of the last periodic-z solve, [2][nx][ny//2+1]
+        (real, imaginary plane); zeros with one element."""
+        a = np.zeros((2, self.nx, self.ny // 2 + 1))
+        check(self.lib.hz_plan_periodic_closure(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return a
+
     def mixed_fp64_modes(self):
         k = ctypes.c_int(0)
         check(self.lib.hz_mixed_fp64_modes(self.handle, ctypes.byref(k)))

@@ -747,7 +747,19 @@ done:
  * closure alpha = -(zl + zr)/(1 + el + er) of the last element combines them
  * (:696-711).  Assembly and the inverse transform as in the non-periodic
  * solver (:854-862).  desc->size must be 1; the Robin fields are unused. */
-int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u)
+static int solve_periodic(const hzo_desc *d, const double *f, double *u, double *alpha_out);
+
+int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u) { return solve_periodic(d, f, u, NULL); }
+
+/* As hzo_solve_periodic, also returning the closure coefficients alphaPDE
+ * (Periodic3D_MPI.cpp:698, [Nx][2 Ny] in the reference's spectral layout;
+ * untouched with one element, where the reference computes none). */
+int hzo_solve_periodic_alpha(const hzo_desc *d, const double *f, double *u, double *alpha)
+{
+    return solve_periodic(d, f, u, alpha);
+}
+
+static int solve_periodic(const hzo_desc *d, const double *f, double *u, double *alpha_out)
 {
     const int Nx = d->Nx, Ny = d->Ny, N = d->N;
     if (!is_pow2(Nx) || !is_pow2(Ny)) return HZO_ERR_NOT_POW2;
@@ -854,6 +866,7 @@ int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u)
             /* closure :698-711 */
             const double alpha = -(PL(s->zetal, e, i, j) + PL(s->zetar, e, i, j)) /
                                  (1.0 + PL(etal, e, i, j) + PL(etar, e, i, j));
+            if (alpha_out) alpha_out[(size_t)i * NY + j] = alpha;
             for (int kk = 0; kk < Ns; ++kk) {
                 PL(s->zetal, kk, i, j) += alpha * PL(etal, kk, i, j);
                 PL(s->zetar, kk, i, j) += alpha * PL(etar, kk, i, j);

@@ -92,6 +92,9 @@ int hzo_solve(const hzo_desc *d, const double *f, const double *gl,
 int hzo_derivative_z(const hzo_desc *d, int order, const double *u, double *out);
 /* Periodic3D_MPI.cpp SolvePDE at size = 1: u[Nx][Ny][Ktot] from f (z periodic). */
 int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u);
+/* The same, also writing alphaPDE[Nx][2 Ny] (the closure coefficient per
+ * spectral column, Periodic3D_MPI.cpp:698) when there is more than one element. */
+int hzo_solve_periodic_alpha(const hzo_desc *d, const double *f, double *u, double *alpha);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,7 @@ def lib(fma=False):
         L.hzo_solve.argtypes = [ctypes.POINTER(Desc), _D, _D, _D, _D]
         L.hzo_derivative_z.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, _D, _D]
         L.hzo_solve_periodic.argtypes = [ctypes.POINTER(Desc), _D, _D]
+        L.hzo_solve_periodic_alpha.argtypes = [ctypes.POINTER(Desc), _D, _D, _D]
         if fma:
             _fma = L
         else:
@@ -166,13 +167,19 @@ def solve(desc, f, gl, gr, fma=False):
     return u
 
 
-def solve_periodic(desc, f, fma=False):
-    """Reference periodic-z SolvePDE (Periodic3D_MPI.cpp) at size = 1."""
+def solve_periodic(desc, f, fma=False, alpha=False):
+    """Reference periodic-z SolvePDE (Periodic3D_MPI.cpp) at size = 1; with
+    alpha=True also its closure coefficients alphaPDE[Nx][2 Ny] (zeros for
+    one element)."""
     K = total_z_points(desc)
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(desc.Nx, desc.Ny, K)
     u = np.zeros_like(f)
-    _check(lib(fma).hzo_solve_periodic(ctypes.byref(desc), _p(f), _p(u)))
-    return u
+    if not alpha:
+        _check(lib(fma).hzo_solve_periodic(ctypes.byref(desc), _p(f), _p(u)))
+        return u
+    a = np.zeros((desc.Nx, 2 * desc.Ny))
+    _check(lib(fma).hzo_solve_periodic_alpha(ctypes.byref(desc), _p(f), _p(u), _p(a)))
+    return u, a
 
 
 def derivative_z(desc, u, order):

@@ -79,3 +79,35 @@ def test_periodic_rejects_unsupported(cuda):
     fbar, glbar, grbar, _ = p.adjoint(torch.zeros((8, 8, p.nz), dtype=torch.float64, device="cuda"))
     assert glbar is None and grbar is None and float(fbar.abs().max()) == 0.0
     p.close()
+
+
+@pytest.mark.parametrize("form", ["plain", "diag", "mixed"])
+def test_periodic_closure_matches_oracle(cuda, form):
+    """The closure coefficients of the cyclic interface system (the
+    reference's alphaPDE) from a three-element periodic solve, expanded to the
+    reference's full-spectrum layout the way the drop-in does, against the
+    oracle's."""
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny, N, ne, lam = 8, 16, 9, 3, 1.5
+    plan = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * np.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam, form=form,
+                         periodic_z=True)
+    desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * np.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam)
+    f = np.random.default_rng(77).standard_normal((nx, ny, plan.nz))
+    plan.solve(dev(f))
+    half = plan.periodic_closure()
+    M = ny // 2 + 1
+    full = np.zeros((nx, 2 * ny))
+    for i in range(nx):
+        for m in range(ny):
+            low = m < M
+            ii, mm = (i, m) if low else ((nx - i) % nx, ny - m)
+            full[i, 2 * m] = half[0, ii, mm]
+            full[i, 2 * m + 1] = half[1, ii, mm] if low else -half[1, ii, mm]
+    _, want = po.solve_periodic(desc, f, alpha=True)
+    assert np.abs(want).max() > 0
+    assert np.abs(full - want).max() <= 1e-10 * np.abs(want).max()
+    one = HelmholtzPlan(8, 8, 9, 1, lam=1.0, form=form, periodic_z=True)
+    one.solve(dev(np.ones((8, 8, one.nz))))
+    assert not one.periodic_closure().any()  # one element: no closure, as in the reference
+    plan.close()
+    one.close()
