@@ -245,7 +245,9 @@ struct Variant {
 // The fused persistent variants are switched off: their first run on the
 // GPU did not finish (its dependency waits ran to the spin limit, i.e. the
 // completion counters written by work-groups on other XCDs were not seen),
-// and the cause is not established.  Only the staged variants run.
+// and the cause is not established.  Only the staged variants run.  (The
+// fused task bodies also predate the length-1 passes of 2-D and 1-D plans:
+// fft_rows has no L = 1 case, so they would need one before being enabled.)
 const Variant kVariants[] = {{32, 1, 2, 0}, {32, 1, 3, 0}, {32, 1, 4, 0}, {64, 1, 2, 0}, {32, 0, 0, 1}, {64, 0, 0, 1}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kSpinLimit = 1 << 16;  // ~0.1 s of 2-us probes, then the error flag
