@@ -170,3 +170,24 @@ def test_probe_ceilings_are_plausible(cuda):
     assert 300.0 < copy.value <= 8000.0 * 1.5  # an Infinity-Cache-resident copy can beat HBM
     with pytest.raises(_lib.HzError):
         _lib.check(lib.hz_probe_ceilings(None, None, 1 << 20))
+
+
+@pytest.mark.parametrize("nx,ny", [(1024, 16), (16, 1024), (512, 64)])
+@pytest.mark.parametrize("form", ["diag", "plain"])
+def test_largest_transform_lengths_solve(cuda, nx, ny, form):
+    """Solves whose transforms run at the longest lengths (1024 and 512
+    points: the two-stage radix kernels, on the padded spectrum rows),
+    against the oracle."""
+    from mpi_amd.solver import HelmholtzPlan
+    N = 9
+    plan = HelmholtzPlan(nx, ny, N, 1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1, lam=2.0,
+                         alpha_l=0.3, beta_l=1.0, alpha_r=0.2, beta_r=1.0, form=form)
+    rng = np.random.default_rng(nx + 3 * ny)
+    f = rng.standard_normal((nx, ny, plan.nz))
+    gl = rng.standard_normal((nx, ny))
+    gr = rng.standard_normal((nx, ny))
+    u = plan.solve(to_dev(f), to_dev(gl), to_dev(gr)).cpu().numpy()
+    desc = po.make_desc(nx, ny, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
+                        alphal=0.3, betal=1.0, alphar=0.2, betar=1.0, lam=2.0)
+    assert rel_err(u, po.solve(desc, f, gl, gr)) <= (1e-12 if form == "plain" else TOL)
+    plan.close()
