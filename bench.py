@@ -248,7 +248,26 @@ def time_allen_cahn():
             "loss_and_grad_ms": 1e3 * (t2 - t1), "loss": L, "dL_deps": deps}
 
 
-def time_batch(steps=20):
+def batch_roofline(plans, batch_ms, ceilings):
+    """Ideal time of the batch at the measured ceilings: the mode solve's
+    padded matrix work (two contractions of N-2 rounded up to 32 per real
+    column) at the FP64 MFMA rate plus the four transform passes' bytes
+    (field and half spectrum read and written once each) at the 2 GiB copy
+    rate (the batch streams from HBM), over the measured ms."""
+    flop = byts = 0.0
+    for p in plans:
+        M = p.ny // 2 + 1
+        NP = 32 * ((p.nz - 2 + 31) // 32)
+        flop += 2 * 2 * 2.0 * NP * NP * p.nx * M
+        real, half = 8.0 * p.nx * p.ny * p.nz, 16.0 * p.nx * M * p.nz
+        byts += 2 * (real + half) + 2 * (2 * half)
+    t_mode = flop / (ceilings["fp64_mfma_tflops"] * 1e12) * 1e3
+    t_fft = byts / (ceilings["copy_gbps_2GiB"] * 1e9) * 1e3
+    return {"mode_solve_flop": flop, "transform_bytes": byts, "ideal_ms": t_mode + t_fft,
+            "frac_of_attainable": (t_mode + t_fft) / batch_ms}
+
+
+def time_batch(steps=20, ceilings=None):
     """Many-problem workload: 512 independent problems, nx, ny uniform in
     {32, 64, 128}, N odd uniform in [17, 129], lambda log-uniform in
     [1e-2, 1e3], 16 random modes each; one hz_batch (autotuned, graph
@@ -291,6 +310,8 @@ def time_batch(steps=20):
            "points": int(sum(p.nx * p.ny * p.nz for p in plans)), "batch_ms": batch_ms,
            "problems_per_s": 512 / (batch_ms * 1e-3), "one_plan_at_a_time_ms": seq_ms,
            "speedup": seq_ms / batch_ms, "tuning": b.tuning()}
+    if ceilings:
+        out["roofline"] = batch_roofline(plans, batch_ms, ceilings)
     b.close()
     for p in plans:
         p.close()
@@ -475,7 +496,7 @@ def main():
     if not args.no_extras and "diag" in results:
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
         line["allen_cahn"] = time_allen_cahn()
-        line["batch"] = time_batch()
+        line["batch"] = time_batch(ceilings=roof["attainable"])
         line["precision"] = time_precision()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(f, gl, gr)
