@@ -98,7 +98,11 @@ class AllenCahn:
         uT = u
         L = self.loss(uT, target)
         ubar = 2.0 * self.W * (uT - target)
-        eps_bar = 0.0
+        # dL/deps accumulates on the device (one host read at the end):
+        # per step dL/dlam = sum(fbar * u_out) (du/dlam = S(u, 0, 0)) times
+        # dlam/deps, plus the explicit eps dependence of the right-hand side
+        eps_bar = torch.zeros((), dtype=torch.float64, device="cuda")
+        inv_e2 = 1.0 / (eps * eps)
         for s in sorted(ckpt, reverse=True):
             e = min(s + self.every, self.steps)
             states = [ckpt[s]]
@@ -106,12 +110,11 @@ class AllenCahn:
                 states.append(self.step(states[-1], eps, lam))
             for t in range(e - 1, s - 1, -1):
                 u_in, u_out = states[t - s], states[t - s + 1]
-                fbar, _, _, lam_bar = self.plan.adjoint(ubar, u_out)
-                inv_e2 = 1.0 / (eps * eps)
-                eps_bar += lam_bar * dlam
-                eps_bar += float(torch.sum(fbar * (-dlam * u_in - (2.0 / eps ** 3) * (u_in * u_in * u_in - u_in))))
+                fbar, _, _, _ = self.plan.adjoint(ubar)
+                eps_bar = eps_bar + torch.sum(fbar * u_out) * dlam
+                eps_bar = eps_bar + torch.sum(fbar * (-dlam * u_in - (2.0 / eps ** 3) * (u_in * u_in * u_in - u_in)))
                 ubar = fbar * (-lam + (3.0 * u_in * u_in - 1.0) * inv_e2)
-        return L, ubar, eps_bar
+        return L, ubar, float(eps_bar)
 
     def close(self):
         self.plan.close()
