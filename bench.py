@@ -15,8 +15,8 @@ Forms timed (libhz.so through its C ABI):
 The reported `value` is the chosen form's rate (the fastest form whose
 error against the oracle is <= 1e-10); every form's rate and error is listed.
 
-Prints one JSON line (rank 0).  --gpus > 1 is refused: this build targets one
-GPU (see DESIGN.md for how the path shards).
+Prints one JSON line (rank 0).  --gpus > 1 is refused: the solver runs on
+one GPU; sharding the grid over several GPUs is a design, not built code.
 """
 import argparse
 import json
@@ -443,8 +443,8 @@ def main():
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "the solver runs on one GPU; --gpus must be 1 "
+                                   "(sharding over several GPUs is designed, not built)"}))
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
