@@ -2,14 +2,20 @@
 include/hz.h declares, and its host-side plan setup is bit-identical to the
 reference (golden vectors) -- no compute calls on a GPU."""
 import ctypes
+import json
+import math
+import os
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
 
 from golden_util import load, sha
 
-HEADER = __import__("os").path.join(__import__("os").path.dirname(__file__), "..", "include", "hz.h")
+ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..")
+HEADER = os.path.join(ROOT, "include", "hz.h")
 
 
 def declared_symbols():
@@ -71,4 +77,12 @@ def test_host_eigendecomposition(N):
     assert abs(mu[-1] + (math.pi / 2) ** 2) < 1e-6
 
 
-import math  # noqa: E402
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus 2 exits with status 2 and one JSON line naming the
+    reason, before it touches a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
