@@ -157,6 +157,9 @@ int launch_spectral_derivative(double *re, double *im, long long A, long long B,
                                double len, int order, hipStream_t s);
 // sum_i a[i]*b[i] into *out (device), deterministic two-pass reduction; partial >= 1024 doubles
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
+// n pseudo-random doubles in [-0.5, 0.5) (a hash of the index and seed):
+// timing data for the plan-time tuners.
+int launch_fill_noise(double *p, size_t n, unsigned long long seed, hipStream_t s);
 
 }  // namespace hz
 

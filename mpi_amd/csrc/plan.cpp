@@ -440,11 +440,17 @@ int tune_diag(hz_plan *p)
 {
     int choices[8];
     const int k = diag_tile_choices(p->NP, choices, 8);
-    HZ_HIP(hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->xs));
-    HZ_HIP(hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M));
+    const size_t nspec = 2 * (size_t)p->nx * p->xs, nbc = 4 * (size_t)p->nx * p->M;
     hipStream_t s;
     hipEvent_t e0, e1;
     HZ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    // timed on noise (zeros run faster); the buffers are cleared afterwards
+    int fst = launch_fill_noise(p->spec.re, nspec, 1, s);
+    if (!fst) fst = launch_fill_noise(p->bc.re, nbc, 2, s);
+    if (fst) {
+        (void)hipStreamDestroy(s);
+        return fst;
+    }
     HZ_HIP(hipEventCreate(&e0));
     HZ_HIP(hipEventCreate(&e1));
     double best = 1e300;
@@ -467,9 +473,13 @@ int tune_diag(hz_plan *p)
             p->diag_rt = choices[i];
         }
     }
+    if (hipStreamSynchronize(s) != hipSuccess && !st) st = set_error(HZ_ERR_HIP, "tuning run failed");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
+    if (!st && (hipMemset(p->spec.re, 0, sizeof(double) * nspec) != hipSuccess ||
+                hipMemset(p->bc.re, 0, sizeof(double) * nbc) != hipSuccess))
+        st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
     if (!st && best == 1e300)
         st = set_error(HZ_ERR_UNSUPPORTED, "DIAG form: 2*ne must fit one tile (ne <= 64 when N-2 <= 64, "
                                            "32 when N-2 <= 128, 16 when N-2 <= 256)");
@@ -487,15 +497,16 @@ int tune_fft(hz_plan *p)
     if (hipMalloc((void **)&in, sizeof(double) * phys) != hipSuccess)
         return set_error(HZ_ERR_ALLOC, "hipMalloc: tuning buffer");
     int st = HZ_OK;
-    if (hipMemset(in, 0, sizeof(double) * phys) != hipSuccess ||
-        hipMemset(p->spec.re, 0, sizeof(double) * 2 * (size_t)p->nx * p->xs) != hipSuccess ||
-        hipMemset(p->bc.re, 0, sizeof(double) * 4 * (size_t)p->nx * p->M) != hipSuccess)
-        st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
+    const size_t nspec = 2 * (size_t)p->nx * p->xs, nbc = 4 * (size_t)p->nx * p->M;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (!st && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
         st = set_error(HZ_ERR_HIP, "tuning stream failed");
+    // timed on noise (zeros run faster); the plan's buffers are cleared afterwards
+    if (!st) st = launch_fill_noise(in, phys, 3, s);
+    if (!st) st = launch_fill_noise(p->spec.re, nspec, 4, s);
+    if (!st) st = launch_fill_noise(p->bc.re, nbc, 5, s);
     static const FftKind kinds[4] = {FFT_R2C, FFT_C2C_FWD, FFT_C2C_INV, FFT_C2R};
     for (int pass = 0; pass < 4 && !st; ++pass) {
         const int L = (pass == 0 || pass == 3) ? p->ny : p->nx;
@@ -530,10 +541,14 @@ int tune_fft(hz_plan *p)
         p->fft_w[pass] = bestw;
         p->fft_w_ms[pass] = k ? best / 3.0 : 0.0;
     }
+    if (s && hipStreamSynchronize(s) != hipSuccess && !st) st = set_error(HZ_ERR_HIP, "tuning run failed");
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (s) (void)hipStreamDestroy(s);
     (void)hipFree(in);
+    if (!st && (hipMemset(p->spec.re, 0, sizeof(double) * nspec) != hipSuccess ||
+                hipMemset(p->bc.re, 0, sizeof(double) * nbc) != hipSuccess))
+        st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
     return st;
 }
 
@@ -631,12 +646,13 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
-    if (p->d_closure) HZ_HIP(hipMemset(p->d_closure, 0, sizeof(double) * 2 * (size_t)p->nx * p->M));
     if ((d.form == HZ_FORM_PLAIN_LU && (st = lu_modes_check(p->nx, p->n, p->ne))) || (st = tune_fft(p)) ||
         (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
         free_plan(p);
         return st;
     }
+    // after tuning: the tuned DIAG runs of a periodic plan write closures
+    if (p->d_closure) HZ_HIP(hipMemset(p->d_closure, 0, sizeof(double) * 2 * (size_t)p->nx * p->M));
     *plan = p;
     return HZ_OK;
 }
