@@ -120,6 +120,34 @@ def test_lambda_gradient_matches_finite_differences(cuda):
     plan.close()
 
 
+@pytest.mark.parametrize("nx,N,offset", [(64, 65, 0), (64, 65, 1), (1, 33, 0)])
+def test_lambda_gradient_is_the_dot_of_fbar_and_u(cuda, nx, N, offset):
+    """dL/dlambda = sum(fbar * u) to round-off, with u 16-byte aligned, u
+    shifted by one element, and an odd element count (a 1 x 1 x 33 plan)."""
+    import torch
+    from mpi_amd import _lib
+    plan = make_plan(nx, nx, N, ROBIN, 3.0, "diag", yr=2 * math.pi)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    K = plan.nz
+    f = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    gl = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    gr = torch.randn((nx, nx), dtype=torch.float64, device="cuda", generator=g)
+    ubar = torch.randn((nx, nx, K), dtype=torch.float64, device="cuda", generator=g)
+    n = nx * nx * K
+    store = torch.empty(n + offset, dtype=torch.float64, device="cuda")
+    u = store[offset:].view(nx, nx, K)
+    plan.solve(f, gl, gr, out=u)
+    try:
+        fbar, _, _, lam_bar = plan.adjoint(ubar, u)
+    except _lib.HzError as e:
+        if e.code == _lib.HZ_ERR_UNSUPPORTED:
+            pytest.skip(f"adjoint not offered for this plan: {e}")
+        raise
+    prod = (fbar * u).flatten()
+    assert abs(lam_bar - float(torch.sum(prod))) <= 1e-13 * float(torch.sum(prod.abs()))
+    plan.close()
+
+
 def test_autograd_gradcheck(cuda):
     import torch
     from mpi_amd.autograd import helmholtz_solve
