@@ -57,7 +57,7 @@ def oracle_matrix(nx, ny, N, bc, lam, yr=3.0, ne=1):
     return S, K
 
 
-@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("form", ["diag", "plain", "mixed"])
 @pytest.mark.parametrize("bc", [DIRICHLET, ROBIN, NEUMANN], ids=["dirichlet", "robin", "neumann"])
 def test_adjoint_equals_oracle_transpose(cuda, form, bc):
     nx, ny, N, lam = 4, 8, 9, 3.0
@@ -163,7 +163,7 @@ def test_autograd_gradcheck(cuda):
     plan.close()
 
 
-@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("form", ["diag", "plain", "mixed"])
 @pytest.mark.parametrize("bc", [DIRICHLET, ROBIN, NEUMANN], ids=["dirichlet", "robin", "neumann"])
 @pytest.mark.parametrize("ne", [2, 3])
 def test_multi_element_adjoint_equals_oracle_transpose(cuda, form, bc, ne):
@@ -271,7 +271,7 @@ def test_allen_cahn_checkpointing_is_exact(cuda):
     ac.close()
 
 
-@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("form", ["diag", "plain", "mixed"])
 def test_periodic_adjoint_equals_oracle_transpose(cuda, form):
     """Periodic z (no face data): the GPU adjoint equals S^T ubar with S the
     oracle's periodic solve map, column by column; then the dot-product
@@ -330,7 +330,7 @@ def test_periodic_adjoint_equals_oracle_transpose(cuda, form):
     small.close()
 
 
-@pytest.mark.parametrize("form", ["diag", "plain"])
+@pytest.mark.parametrize("form", ["diag", "plain", "mixed"])
 @pytest.mark.parametrize("ne", [2, 3, 4])
 def test_periodic_multi_element_adjoint_equals_oracle_transpose(cuda, form, ne):
     """Periodic z with several elements: the transposed cyclic interface
