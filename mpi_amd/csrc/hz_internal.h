@@ -155,7 +155,8 @@ int launch_derivative(const double *D, int N, long long ncols, const double *u, 
 // axis A (axis_a) or B of length L (Fourier derivative).
 int launch_spectral_derivative(double *re, double *im, long long A, long long B, long long lanes, int axis_a, int L,
                                double len, int order, hipStream_t s);
-// sum_i a[i]*b[i] into *out (device), deterministic two-pass reduction; partial >= 1024 doubles
+// sum_i a[i]*b[i] into *out (device or pinned host memory), deterministic two-pass reduction;
+// partial >= 1024 doubles
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
 // n pseudo-random doubles in [-0.5, 0.5) (a hash of the index and seed):
 // timing data for the plan-time tuners.
@@ -208,6 +209,7 @@ struct hz_plan {
     double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass
     std::vector<double> diag_rt_ms;                   // tuning times per choice
     double *d_partial = nullptr;                      // reduction scratch
+    double *h_lambda_bar = nullptr;                   // pinned host word the dL/dlambda reduction writes
     double *d_xspec = nullptr;                        // half spectrum of the x/y derivatives
     size_t xspec_n = 0;
     // MIXED form

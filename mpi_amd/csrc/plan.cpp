@@ -83,6 +83,7 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->d_slot);
     (void)hipFree(p->d_xspec);
     (void)hipFree(p->d_status);
+    if (p->h_lambda_bar) (void)hipHostFree(p->h_lambda_bar);
     delete p;
 }
 
@@ -891,11 +892,17 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
     if ((st = inverse_xy(p, p->spec.re, p->spec.im, fbar, p->K, s, glbar, grbar, p->xs))) return st;
     rec.mark("fft_inverse");
     if (lambda_bar) {
-        if (!p->d_partial && (st = dmalloc(&p->d_partial, 1025))) return st;
-        if ((st = launch_dot(fbar, u, (long long)p->nx * p->ny * p->K, p->d_partial, p->d_partial + 1024, s)))
+        if (!p->d_partial && (st = dmalloc(&p->d_partial, 1024))) return st;
+        // the final reduction writes straight into pinned host memory (no
+        // separate device-to-host copy before the wait)
+        if (!p->h_lambda_bar && hipHostMalloc((void **)&p->h_lambda_bar, sizeof(double), hipHostMallocCoherent) != hipSuccess) {
+            p->h_lambda_bar = nullptr;
+            return set_error(HZ_ERR_ALLOC, "hipHostMalloc: dL/dlambda word");
+        }
+        if ((st = launch_dot(fbar, u, (long long)p->nx * p->ny * p->K, p->d_partial, p->h_lambda_bar, s)))
             return st;
-        HZ_HIP(hipMemcpyAsync(lambda_bar, p->d_partial + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
         HZ_HIP(hipStreamSynchronize(s));
+        *lambda_bar = *(volatile double *)p->h_lambda_bar;
         rec.mark("lambda_dot");
     }
     return HZ_OK;
