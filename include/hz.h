@@ -140,10 +140,17 @@ int hz_solve_status(hz_plan *plan, hz_stream_t stream);
  *   glbar = dL/dg_l, grbar = dL/dg_r   [nx][ny]
  * and, if u != NULL (the forward solution) and lambda_bar != NULL (host),
  *   *lambda_bar = dL/dlambda = sum over nodes of fbar * u.
- * DIAG and PLAIN_LU forms, any number of elements; with periodic z there
- * is no face data and glbar, grbar must be NULL. */
+ * PLAIN_LU, DIAG and MIXED forms, any number of elements; with periodic z
+ * there is no face data and glbar, grbar must be NULL. */
 int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
                double *glbar, double *grbar, double *lambda_bar, hz_stream_t stream);
+
+/* No counterpart in the reference: the explicit right-hand side of a
+ * semi-implicit Allen-Cahn step, f[i] = a3 u[i]^3 + a1 u[i] for i < n
+ * (device pointers; f may equal u), computed as u * (a3 u u + a1) in one
+ * pass on the caller's stream.  mpi_amd/allen_cahn.py calls it with
+ * a3 = 1/eps^2, a1 = -(1/eps^2 + lambda). */
+int hz_cubic_rhs(const double *u, double *f, long long n, double a3, double a1, hz_stream_t stream);
 
 /* Replaces FirstDerivativeZ / SecondDerivativeZ (NonPeriodic3D_MPI.cpp:632-683):
  * out = Dz u (order 1) or Dzz u (order 2) applied to every element column.

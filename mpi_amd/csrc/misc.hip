@@ -41,6 +41,16 @@ __global__ __launch_bounds__(256) void dot_partial_kernel(const double *__restri
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// f = a3 u^3 + a1 u, one element per thread (one read, one write).
+__global__ __launch_bounds__(256) void cubic_kernel(const double *__restrict__ u, double *__restrict__ f, long long n,
+                                                    double a3, double a1)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double x = u[i];
+    f[i] = x * fma(a3 * x, x, a1);
+}
+
 __global__ __launch_bounds__(256) void dot_final_kernel(const double *__restrict__ partial, int np, double *out)
 {
     __shared__ double red[256];
@@ -119,6 +129,17 @@ int launch_derivative(const double *D, int N, long long ncols, const double *u, 
 }  // namespace hz
 
 using namespace hz;
+
+extern "C" int hz_cubic_rhs(const double *u, double *f, long long n, double a3, double a1, hz_stream_t stream)
+{
+    if (n < 0 || (n > 0 && (!u || !f))) return set_error(HZ_ERR_ARG, "null argument or negative size");
+    if (n == 0) return HZ_OK;
+    const long long grid = (n + 255) / 256;
+    if (grid > 0x7fffffffLL) return set_error(HZ_ERR_ARG, "size too large");
+    hipLaunchKernelGGL(cubic_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, u, f, n, a3, a1);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
 
 extern "C" int hz_derivative_z(hz_plan *p, int order, const double *u, double *out, hz_stream_t stream)
 {

@@ -377,3 +377,28 @@ def test_allen_cahn_graph_replays_match_eager(cuda):
     torch.cuda.synchronize()
     assert torch.equal(u_out, ac.run(u1, 0.02))
     ac.close()
+
+
+@pytest.mark.parametrize("n", [0, 1, 255, 1000003])
+def test_cubic_rhs_matches_torch(cuda, n):
+    """hz_cubic_rhs: f = a3 u^3 + a1 u against the same expression in torch
+    to a few roundings of the terms' magnitudes; in place (f = u) gives the
+    same bits; n = 0 is a no-op; a null operand is refused."""
+    import torch
+    from mpi_amd import _lib
+    lib = _lib.load()
+    g = torch.Generator(device="cuda").manual_seed(n)
+    u = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    a3, a1 = 2500.0, -(2500.0 + 1e5)
+    f = torch.empty_like(u)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.hz_cubic_rhs(u.data_ptr(), f.data_ptr(), n, a3, a1, s))
+    want = a3 * u ** 3 + a1 * u
+    scale = a3 * u.abs() ** 3 + abs(a1) * u.abs()
+    # about three roundings on either side: |difference| <= 6 u scale, u = 2^-53
+    assert bool(torch.all((f - want).abs() <= 1e-15 * scale + 1e-300))
+    v = u.clone()
+    _lib.check(lib.hz_cubic_rhs(v.data_ptr(), v.data_ptr(), n, a3, a1, s))
+    assert torch.equal(v, f)
+    with pytest.raises(_lib.HzError):
+        _lib.check(lib.hz_cubic_rhs(None, f.data_ptr(), 5, a3, a1, s))
