@@ -33,6 +33,9 @@ void CreatePDEPlan(const int Nx, const int Ny, const int N, const double xl, con
                    const double yr);
 void DestroyPDEPlan();
 
+// The particular solution of one column in every element, from rhsPDE with
+// the operator the caller factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[], const int i);
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int));
 
 // The reference leaves the X versions empty (NonPeriodic2D_MPI.cpp:479-492);

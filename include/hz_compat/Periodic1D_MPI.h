@@ -31,6 +31,9 @@ extern int SolverForm;
 void CreatePDEPlan(const int Nx, const int N, const double xl, const double xr);
 void DestroyPDEPlan();
 
+// The particular solution of one column in every element, from rhsPDE with
+// the operator the caller factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[]);
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)());
 
 void FirstDerivativeX(FieldVariable Ux[], FieldVariable U[]);

@@ -14,8 +14,12 @@
 //    reference computes none); its values agree to rounding, not bit for bit;
 //  - `determinant` is not updated while SolvePDE records operators;
 //  - SolverForm selects the GPU method (HZ_FORM_PLAIN_LU, the reference
-//    algorithm, by default; HZ_FORM_DIAG for the z-diagonalised form when the
-//    callback builds the standard shifted operator).
+//    algorithm, by default; HZ_FORM_DIAG for the z-diagonalised form, used
+//    when the callback builds the standard shifted operator to the bit; any
+//    other operator runs the reference algorithm).  A PLAIN_LU plan that the
+//    reference-algorithm kernel cannot hold (N - 2 > 128, or too many
+//    elements) is created in the DIAG form, and its solves then need the
+//    standard operator.
 #pragma once
 
 #include "../hz.h"
@@ -38,6 +42,9 @@ void CreatePDEPlan(const int Nx, const int Ny, const int Nz, const int N, const 
                    const double yl, const double yr, const double zl, const double zr);
 void DestroyPDEPlan();
 
+// The particular solution of one column in every element, from rhsPDE with
+// the operator the caller factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[], const int i, const int j);
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int, const int));
 
 // The reference leaves the X/Y versions empty (Periodic3D_MPI.cpp:869-900);

@@ -95,7 +95,10 @@ class AllenCahn:
         (torch.cuda.CUDAGraph; the solver's kernels go on the capturing
         stream) and returns (replay, u_in, u_out): replay() runs all steps,
         reading u_in and leaving u_T in u_out.  Replays recompute from u_in,
-        so refill u_in (copy_) to start from a new state."""
+        so refill u_in (copy_) to start from a new state.  The graph holds
+        this eps (the right-hand side's coefficients are recorded in it):
+        replay() raises once the plan's lambda has changed, e.g. by a run at
+        another eps; capture again then."""
         lam = self._set_eps(eps)
         u_in = u0.contiguous().clone()
         side = torch.cuda.Stream()
@@ -110,7 +113,13 @@ class AllenCahn:
             u = u_in
             for _ in range(self.steps):
                 u = self.step(u, eps, lam)
-        return graph.replay, u_in, u
+
+        def replay():
+            if self.plan.desc.lam != lam:
+                raise RuntimeError(f"graph recorded at eps = {eps!r}; the plan's lambda has changed since "
+                                   "(capture again)")
+            graph.replay()
+        return replay, u_in, u
 
     def loss(self, uT, target):
         d = uT - target

@@ -107,6 +107,20 @@ void DestroyPDEPlan()
     g.destroy();
 }
 
+// NonPeriodic1D_MPI.cpp:144-190, Periodic1D_MPI.cpp:140-186: the particular
+// solution in every element from rhsPDE, with the operator the caller
+// factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[])
+{
+    std::vector<double *> cols(Ns);
+    std::vector<const double *> rhs(Ns);
+    for (int jj = 0; jj < Ns; ++jj) {
+        cols[jj] = U[jj].phi;
+        rhs[jj] = rhsPDE[jj];
+    }
+    hzc::particular_solutions(cols.data(), rhs.data(), Ns, N, Operator, npivot);
+}
+
 // NonPeriodic1D_MPI.cpp:195-355, Periodic1D_MPI.cpp:191-518
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)())
 {

@@ -161,6 +161,20 @@ void DestroyPDEPlan()
     g.destroy();
 }
 
+// NonPeriodic2D_MPI.cpp:176-223, Periodic2D_MPI.cpp:172-219: the particular
+// solution of column i in every element from rhsPDE, with the operator the
+// caller factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[], const int i)
+{
+    std::vector<double *> cols(Ns);
+    std::vector<const double *> rhs(Ns);
+    for (int jj = 0; jj < Ns; ++jj) {
+        cols[jj] = U[jj].phi[i];
+        rhs[jj] = rhsPDE[jj][i];
+    }
+    hzc::particular_solutions(cols.data(), rhs.data(), Ns, N, Operator, npivot);
+}
+
 // NonPeriodic2D_MPI.cpp:228-474, Periodic2D_MPI.cpp:224-691
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int))
 {

@@ -174,6 +174,20 @@ void DestroyPDEPlan()
     g.destroy();
 }
 
+// NonPeriodic3D_MPI.cpp:184-232, Periodic3D_MPI.cpp:180-228: the particular
+// solution of column (i, j) in every element from rhsPDE, with the operator
+// the caller factored into Operator / npivot.
+void ConstructFundamentalSolutions(FieldVariable U[], const int i, const int j)
+{
+    std::vector<double *> cols(Ns);
+    std::vector<const double *> rhs(Ns);
+    for (int kk = 0; kk < Ns; ++kk) {
+        cols[kk] = U[kk].phi[i][j];
+        rhs[kk] = rhsPDE[kk][i][j];
+    }
+    hzc::particular_solutions(cols.data(), rhs.data(), Ns, N, Operator, npivot);
+}
+
 // NonPeriodic3D_MPI.cpp:237-591, Periodic3D_MPI.cpp:233-864
 void SolvePDE(FieldVariable U[], void (*ConstructOperator)(const int, const int))
 {

@@ -5,11 +5,13 @@
 // Neumann data on the first z face, Dirichlet on the last, lambda = 1, grid
 // 64 x 32 with Nz = 250 requested z points of N = 10 per element.
 //
-//   TestNonPeriodic3D [callback]   callback = shift (default) | full | singular
+//   TestNonPeriodic3D [callback]   callback = shift (default) | full | singular | diag
 // "full" changes an off-diagonal entry of the operator of mode (1, 2), whose
 // right-hand side is zero for this problem, so SolvePDE must record whole
 // matrices while the solution is unchanged; "singular" zeroes a column of
-// the (0, 0) operator, which the reference reports as singular.
+// the (0, 0) operator, which the reference reports as singular; "diag" sets
+// SolverForm = HZ_FORM_DIAG before CreatePDEPlan (the standard operator, so
+// the z-diagonalised solve runs).
 #include "../../include/hz_compat/NonPeriodic3D_MPI.h"
 
 #include <chrono>
@@ -58,6 +60,7 @@ int main(int argc, char **argv)
 {
     if (argc > 1 && !std::strcmp(argv[1], "full")) g_mode = 1;
     if (argc > 1 && !std::strcmp(argv[1], "singular")) g_mode = 2;
+    if (argc > 1 && !std::strcmp(argv[1], "diag")) SolverForm = HZ_FORM_DIAG;
     std::cout.flags(std::ios::dec);
     std::cout.precision(8);
 

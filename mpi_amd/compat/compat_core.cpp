@@ -76,7 +76,13 @@ bool Recorder::take(double **A, int n_)
 
 void Solver::create(const hz_plan_desc &d)
 {
-    hz_check(hz_plan_create(&d, &plan));
+    desc = d;
+    int st = hz_plan_create(&desc, &plan);
+    if (st == HZ_ERR_UNSUPPORTED && desc.form == HZ_FORM_PLAIN_LU) {
+        desc.form = HZ_FORM_DIAG;
+        st = hz_plan_create(&desc, &plan);
+    }
+    hz_check(st);
     hz_check(hz_plan_nz_total(plan, &K));
     hz_check(hz_plan_num_modes(plan, &modes));
     nx = d.nx;
@@ -116,15 +122,36 @@ void Solver::solve_recorded(double lambda, double al, double bl, double ar, doub
         hip_check(hipMemcpy(d_gr, h_g + ng, sizeof(double) * ng, hipMemcpyHostToDevice), "hipMemcpy");
     }
     const Recorder &r = g_rec;
-    const std::vector<double> &ops = r.full ? r.ops : r.diag;
-    (void)hipFree(d_ops);
-    d_ops = nullptr;
-    hip_check(hipMalloc(&d_ops, sizeof(double) * ops.size()), "hipMalloc");
-    hip_check(hipMemcpy(d_ops, ops.data(), sizeof(double) * ops.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    hz_check(hz_solve_operators(plan, d_f, d_gl, d_gr, r.full ? nullptr : d_ops, r.full ? d_ops : nullptr, d_u,
-                                nullptr));
+    if (desc.form != HZ_FORM_PLAIN_LU && recorded_standard(lambda)) {
+        hz_check(hz_solve(plan, d_f, d_gl, d_gr, d_u, nullptr));
+    } else {
+        const std::vector<double> &ops = r.full ? r.ops : r.diag;
+        (void)hipFree(d_ops);
+        d_ops = nullptr;
+        hip_check(hipMalloc(&d_ops, sizeof(double) * ops.size()), "hipMalloc");
+        hip_check(hipMemcpy(d_ops, ops.data(), sizeof(double) * ops.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        hz_check(hz_solve_operators(plan, d_f, d_gl, d_gr, r.full ? nullptr : d_ops, r.full ? d_ops : nullptr, d_u,
+                                    nullptr));
+    }
     if (hz_solve_status(plan, nullptr) == HZ_ERR_SINGULAR) ErrorMessage("ERROR: Matrix is singular!");
     hip_check(hipMemcpy(h_f, d_u, sizeof(double) * nf, hipMemcpyDeviceToHost), "hipMemcpy");
+}
+
+bool Solver::recorded_standard(double lambda) const
+{
+    const Recorder &r = g_rec;
+    if (r.full || r.n != N - 2 || (int)r.diag.size() != modes * r.n) return false;
+    const double PI = 3.1415926535897932;  // include/General.h:14
+    const double Lx = desc.xr - desc.xl, Ly = desc.yr - desc.yl;
+    const int M = ny / 2 + 1;
+    for (int iw = 0; iw <= nx / 2; ++iw)
+        for (int jw = 0; jw < M; ++jw) {
+            const double sigma = lambda + 4.0 * PI * PI * iw * iw / (Lx * Lx) + 4.0 * PI * PI * jw * jw / (Ly * Ly);
+            const double *dg = &r.diag[((size_t)iw * M + jw) * r.n];
+            for (int i = 0; i < r.n; ++i)
+                if (dg[i] != r.dxx[i + 1][i + 1] - sigma) return false;
+        }
+    return true;
 }
 
 void Solver::derivative_z(std::vector<double> &elements, int order)
@@ -167,6 +194,36 @@ void Solver::closure_full(std::vector<double> &alpha)
             alpha[(size_t)i * 2 * ny + 2 * m] = half[h];
             alpha[(size_t)i * 2 * ny + 2 * m + 1] = low ? half[(size_t)nx * M + h] : -half[(size_t)nx * M + h];
         }
+}
+
+void particular_solutions(double *const *cols, const double *const *rhs, int count, int N, double **A, int *piv)
+{
+    const int n = N - 2;
+    if (count <= 0 || n < 1) return;
+    const size_t nn = (size_t)n * n, np = n > 1 ? (size_t)(n - 1) : 1;
+    std::vector<double> hA(nn * count), hb((size_t)n * count);
+    std::vector<int> hp(np * count, 0);
+    for (int c = 0; c < count; ++c) {
+        for (int i = 0; i < n; ++i) std::memcpy(&hA[c * nn + (size_t)i * n], A[i], sizeof(double) * n);
+        if (n > 1) std::memcpy(&hp[c * np], piv, sizeof(int) * (n - 1));
+        std::memcpy(&hb[(size_t)c * n], rhs[c] + 1, sizeof(double) * n);
+    }
+    double *dA = nullptr, *db = nullptr;
+    int *dp = nullptr;
+    hip_check(hipMalloc(&dA, sizeof(double) * hA.size()), "hipMalloc");
+    hip_check(hipMalloc(&db, sizeof(double) * hb.size()), "hipMalloc");
+    hip_check(hipMalloc(&dp, sizeof(int) * hp.size()), "hipMalloc");
+    hip_check(hipMemcpy(dA, hA.data(), sizeof(double) * hA.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(db, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(dp, hp.data(), sizeof(int) * hp.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hz_check(hz_lu_solve_batched(dA, dp, db, n, count, nullptr));
+    hip_check(hipMemcpy(hb.data(), db, sizeof(double) * hb.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    for (void *p : {(void *)dA, (void *)db, (void *)dp}) (void)hipFree(p);
+    for (int c = 0; c < count; ++c) {
+        cols[c][0] = 0.0;
+        cols[c][N - 1] = 0.0;
+        std::memcpy(cols[c] + 1, &hb[(size_t)c * n], sizeof(double) * n);
+    }
 }
 
 }  // namespace hzc

@@ -271,6 +271,7 @@ struct hz_batch {
     int2 *d_list[2] = {nullptr, nullptr};
     std::vector<std::array<int, 3>> groups[2];  // {stage, kind, first}, count in next first
     std::vector<const void *> last;
+    std::vector<unsigned> gens;  // each plan's generation when the tables were built
     std::string shape;
     int variant = -1;
     double variant_ms[kNumVariants] = {};
@@ -479,6 +480,41 @@ int upload_problems(hz_batch *b, const double *const *f, const double *const *gl
     return HZ_OK;
 }
 
+// Autotuning key of a batch: every plan's size and sweep kind.
+std::string batch_shape(const std::vector<hz_plan *> &plans)
+{
+    std::string shape;
+    for (const hz_plan *pl : plans) {
+        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
+        shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +
+                 std::to_string(pl->ne) + (pl->d.periodic_z ? "p;" : general ? "g;" : ";");
+    }
+    return shape;
+}
+
+// Task lists of both tile heights from the plans as they are now (the
+// mode-solve kind of a problem follows its Robin coefficients); the
+// recorded graph and the uploaded table are dropped, so the next solve
+// re-reads every plan.
+int rebuild_lists(hz_batch *b)
+{
+    std::vector<const double *> nil(b->n, nullptr);
+    std::vector<double *> nilo(b->n, nullptr);
+    for (int r = 0; r < 2; ++r) {
+        fill_problems(b, nil.data(), nil.data(), nil.data(), nilo.data(), r ? 64 : 32);
+        if (int st = build_lists(b, r ? 64 : 32)) return st;
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->last.clear();
+    b->gens.resize(b->n);
+    for (int p = 0; p < b->n; ++p) b->gens[p] = b->plans[p]->generation;
+    b->shape = batch_shape(b->plans);
+    return HZ_OK;
+}
+
 void free_batch(hz_batch *b)
 {
     if (!b) return;
@@ -507,7 +543,6 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
     *batch = nullptr;
     int dev = 0;
     HZ_HIP(hipGetDevice(&dev));
-    std::string shape;
     for (int p = 0; p < count; ++p) {
         const hz_plan *pl = plans[p];
         if (!pl) return set_error(HZ_ERR_ARG, "null plan");
@@ -519,15 +554,12 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         if ((pl->nx != 1 && (pl->nx < 16 || pl->nx > 256)) || (pl->ny != 1 && (pl->ny < 16 || pl->ny > 256)))
             return set_error(HZ_ERR_UNSUPPORTED, "batch: nx, ny must be 1 or in [16, 256]");
         if (2 * pl->ne > 32) return set_error(HZ_ERR_UNSUPPORTED, "batch: at most 16 elements");
-        const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
-        shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +
-                 std::to_string(pl->ne) + (pl->d.periodic_z ? "p;" : general ? "g;" : ";");
     }
     hz_batch *b = new hz_batch();
     b->plans.assign(plans, plans + count);
     b->n = count;
     b->device = dev;
-    b->shape = shape;
+    b->shape = batch_shape(b->plans);
     b->nctl = 4 + kStages * count;
     int st = HZ_OK;
     auto fail = [&](int code) {
@@ -558,18 +590,12 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
             hipSuccess)
             return fail(set_error(HZ_ERR_HIP, "batch table upload failed"));
     }
-    {
-        // kinds only depend on the plans: fill the table with null buffers
-        std::vector<const double *> nil(count, nullptr);
-        std::vector<double *> nilo(count, nullptr);
-        for (int r = 0; r < 2; ++r) {
-            fill_problems(b, nil.data(), nil.data(), nil.data(), nilo.data(), r ? 64 : 32);
-            if ((st = build_lists(b, r ? 64 : 32))) return fail(st);
-        }
-    }
+    // kinds only depend on the plans: the lists are built from a table
+    // filled with null buffers
+    if ((st = rebuild_lists(b))) return fail(st);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tuned.find(shape);
+        auto it = g_tuned.find(b->shape);
         if (it != g_tuned.end()) b->variant = it->second;
     }
     *batch = b;
@@ -587,6 +613,14 @@ int hz_batch_solve(hz_batch *b, const double *const *f, const double *const *gl,
     }
     const hipStream_t cs = (hipStream_t)stream;
     int st;
+    // a plan's lambda or Robin coefficients changed since the tables were
+    // built (hz_plan_set_lambda / hz_plan_set_robin): rebuild them
+    for (int p = 0; p < b->n; ++p)
+        if (b->plans[p]->generation != b->gens[p]) {
+            HZ_HIP(hipStreamSynchronize(b->stream));  // a replay may still read the old tables
+            if ((st = rebuild_lists(b))) return st;
+            break;
+        }
     HZ_HIP(hipEventRecord(b->ev_in, cs));
     HZ_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
     std::vector<const void *> ptrs;

@@ -188,6 +188,10 @@ DiagArgs diag_args(const hz_plan *p);
 struct hz_plan {
     hz_plan_desc d{};
     int device = 0;
+    // bumped by every change of lambda or the Robin coefficients, so that a
+    // batch holding copies of them (and the kernel kinds they select) knows
+    // to rebuild its tables
+    unsigned generation = 0;
     int nx = 0, ny = 0, N = 0, n = 0, ne = 0, K = 0, M = 0, modes = 0;
     long long xs = 0;  // x-row stride of spec (spectrum_xs(M, K))
     double Lx = 0, Ly = 0, Lz = 0;

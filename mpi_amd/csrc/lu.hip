@@ -24,7 +24,10 @@
 //
 // Exchanges are implicit: rows never move, every physical row carries its
 // current position, and the pivot row is broadcast from wherever it lives.
-// Elimination uses FMA, so results agree with the reference to round-off.
+// The mode kernel's elimination uses FMA, so its results agree with the
+// reference to round-off.  The standalone factor and solve kernels (the
+// drop-in LUDecomposition / LUSolve) round every product and difference
+// separately, in the reference's order, and reproduce it bit for bit.
 //
 // Layout: 64*TC threads.  Wave c (of TC) owns matrix columns c + TC*b
 // (b < CB) and right-hand sides c + TC*t (t < NR); lane r owns physical rows
@@ -57,6 +60,14 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// a - l*p with the product rounded before the difference
+// (LU_Decomposition.cpp:56-57 `A[l][m] -= A[l][k]*A[k][m]`).
+__device__ __forceinline__ double sub_product_rounded(double a, double l, double p)
+{
+#pragma clang fp contract(off)
+    return a - l * p;
+}
+
 template <int RB, int CB, int NR, int TC>
 struct Smem {
     static constexpr int ROWS = WAVE * RB;
@@ -78,8 +89,9 @@ struct Smem {
 // c + TC*b.  Y[a][t]: same row, right-hand side c + TC*t (nrhs of them).
 // pos[a]: current position of the row.  With out != nullptr the factors are
 // written in the reference's storage (multipliers at their positions, U rows
-// in position order) and the pivots to piv_out.
-template <int RB, int CB, int NR, int TC>
+// in position order) and the pivots to piv_out.  EXACT: rank-1 updates
+// without fused multiply-adds (the reference's rounding).
+template <int RB, int CB, int NR, int TC, bool EXACT = false>
 __device__ __forceinline__ void eliminate(double (&A)[RB][CB], double (&Y)[RB][NR], int (&pos)[RB], int n,
                                           int nrhs, Smem<RB, CB, NR, TC> &s, double *out, int *piv_out)
 {
@@ -271,7 +283,8 @@ __device__ __forceinline__ void eliminate(double (&A)[RB][CB], double (&Y)[RB][N
 #define HZ_UPDATE(b)                                                            \
     if constexpr ((b) < CB) {                                                   \
         const double pv = s.prow[c][(b)];                                       \
-        _Pragma("unroll") for (int a = 0; a < RB; ++a) A[a][(b)] = fma(-l[a], pv, A[a][(b)]); \
+        _Pragma("unroll") for (int a = 0; a < RB; ++a)                          \
+            A[a][(b)] = EXACT ? sub_product_rounded(A[a][(b)], l[a], pv) : fma(-l[a], pv, A[a][(b)]); \
     }
         HZ_SLOTS_FROM(b0, HZ_UPDATE)
 #undef HZ_UPDATE
@@ -549,7 +562,8 @@ __global__ __launch_bounds__(WAVE *TC) void lu_factor_kernel(const LuFactorArgs 
         }
     }
     __syncthreads();  // every read of Am before the factors overwrite it
-    eliminate<RB, CB, 1, TC>(A, Y, pos, n, 0, sm, Am, a.piv ? a.piv + (long long)blockIdx.x * (n - 1) : nullptr);
+    eliminate<RB, CB, 1, TC, true>(A, Y, pos, n, 0, sm, Am,
+                                   a.piv ? a.piv + (long long)blockIdx.x * (n - 1) : nullptr);
     if (threadIdx.x == 0) {
         if (a.det) a.det[blockIdx.x] = sm.det;
         if (a.status) a.status[blockIdx.x] = sm.singular ? HZ_ERR_SINGULAR : 0;
@@ -557,9 +571,11 @@ __global__ __launch_bounds__(WAVE *TC) void lu_factor_kernel(const LuFactorArgs 
 }
 
 // LU_Decomposition.cpp:78-108 for many systems, one thread per system (a
-// utility for callers of the reference's LUSolve; not on the solve path).
+// utility for callers of the reference's LUSolve; not on the solve path),
+// every product and difference rounded separately as in the reference.
 __global__ void lu_solve_kernel(const double *LU, const int *piv, double *b, int n, int count)
 {
+#pragma clang fp contract(off)
     const int sys = blockIdx.x * blockDim.x + threadIdx.x;
     if (sys >= count) return;
     const double *A = LU + (long long)sys * n * n;

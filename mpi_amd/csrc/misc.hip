@@ -4,11 +4,21 @@
 // out[e][i][j][k] = sum_p D[k][p] u[e][i][j][p] for every element column:
 // a batch of small dense products, each thread producing one output node of
 // one column (columns are contiguous, so a wave reads and writes consecutive
-// z values; the N x N matrix stays in the caches).
+// z values; the N x N matrix stays in the caches).  The sum runs over
+// p = 0..N-1 in order with every product rounded before it is added, as in
+// the reference's loop, so the result equals the reference's bit for bit.
 #include "hz_internal.h"
 
 namespace hz {
 namespace {
+
+__device__ __forceinline__ double rounded_dot(const double *__restrict__ a, const double *__restrict__ b, int n)
+{
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc += a[p] * b[p];
+    return acc;
+}
 
 __global__ __launch_bounds__(256) void derivative_kernel(const double *__restrict__ D, int N, long long ncols,
                                                          const double *__restrict__ u, double *__restrict__ out)
@@ -19,9 +29,7 @@ __global__ __launch_bounds__(256) void derivative_kernel(const double *__restric
         const long long col = idx / N;
         const int k = (int)(idx % N);
         const double *uc = u + col * N;
-        double acc = 0.0;
-        for (int p = 0; p < N; ++p) acc = fma(D[k * N + p], uc[p], acc);
-        out[idx] = acc;
+        out[idx] = rounded_dot(D + (long long)k * N, uc, N);
     }
 }
 

@@ -697,6 +697,7 @@ int hz_plan_set_lambda(hz_plan *p, double lambda)
 {
     if (!p) return set_error(HZ_ERR_ARG, "null argument");
     p->d.lambda = lambda;
+    ++p->generation;
     return compute_shifts(p);
 }
 
@@ -707,6 +708,7 @@ int hz_plan_set_robin(hz_plan *p, double alpha_l, double beta_l, double alpha_r,
     p->d.beta_l = beta_l;
     p->d.alpha_r = alpha_r;
     p->d.beta_r = beta_r;
+    ++p->generation;
     return HZ_OK;
 }
 

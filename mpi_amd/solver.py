@@ -75,6 +75,11 @@ class HelmholtzPlan:
         check(self.lib.hz_plan_set_lambda(self.handle, float(lam)))
         self.desc.lam = float(lam)
 
+    def set_robin(self, alpha_l, beta_l, alpha_r, beta_r):
+        check(self.lib.hz_plan_set_robin(self.handle, float(alpha_l), float(beta_l), float(alpha_r), float(beta_r)))
+        self.desc.alpha_l, self.desc.beta_l = float(alpha_l), float(beta_l)
+        self.desc.alpha_r, self.desc.beta_r = float(alpha_r), float(beta_r)
+
     # ---- solves --------------------------------------------------------
     def empty_field(self):
         return torch.empty((self.nx, self.ny, self.nz), dtype=torch.float64, device="cuda")

@@ -903,23 +903,52 @@ done:
 int hzo_derivative_z(const hzo_desc *d, int order, const double *u, double *out)
 {
     const int N = d->N, P = d->size, Ns = hzo_elements_per_rank(d);
-    const int E = P * Ns, Ktot = E * (N - 1) + 1;
+    const int E = P * Ns;
     const double Lz = d->zr - d->zl;
+    if (order != 1 && order != 2) return HZO_ERR_ARG;
     double *Dz = malloc(sizeof(double) * (size_t)N * N), *Dzz = malloc(sizeof(double) * (size_t)N * N);
     if (!Dz || !Dzz) { free(Dz); free(Dzz); return HZO_ERR_ALLOC; }
+    /* every rank scales by the same 2 Ns / Lzp (NonPeriodic3D_MPI.cpp:125-131) */
     int err = hzo_plan_operators(N, Ns, P, 0, Lz, NULL, Dz, Dzz);
     const double *M = order == 1 ? Dz : Dzz;
-    for (size_t col = 0; col < (size_t)d->Nx * d->Ny && !err; ++col) {
-        const double *uc = u + col * Ktot;
-        double *oc = out + col * Ktot;
-        for (int e = 0; e < E; ++e)
-            for (int k = 0; k < N; ++k) {
-                double a = 0.0;
-                for (int p = 0; p < N; ++p) a += M[(size_t)k * N + p] * uc[e * (N - 1) + p];
-                oc[e * (N - 1) + k] = a;  /* interfaces: the later element's value */
-            }
+    const size_t ncol = (size_t)E * d->Nx * d->Ny;
+    for (size_t c = 0; c < ncol && !err; ++c) {
+        const double *uc = u + c * N;
+        double *oc = out + c * N;
+        for (int k = 0; k < N; ++k) {
+            double a = 0.0;
+            for (int p = 0; p < N; ++p) a += M[(size_t)k * N + p] * uc[p];
+            oc[k] = a;
+        }
     }
     free(Dz);
     free(Dzz);
     return err;
+}
+
+int hzo_fundamental_solutions(int N, int Ns, const double *LU, const int *npivot, const double *Dzz,
+                              const double *rhs, double *U, double *ul, double *ur)
+{
+    const int n = N - 2;
+    if (N < 3 || Ns < 1) return HZO_ERR_ARG;
+    for (int kk = 0; kk < Ns; ++kk) {
+        double *u = U + (size_t)kk * N;
+        u[0] = 0.0;
+        u[N - 1] = 0.0;
+        for (int k = 0; k < n; ++k) u[1 + k] = rhs[(size_t)kk * N + k + 1];
+        hzo_lu_solve(LU, npivot, u + 1, n);
+    }
+    if (ul) {
+        ul[0] = 1.0;
+        ul[N - 1] = 0.0;
+        for (int k = 0; k < n; ++k) ul[1 + k] = -Dzz[(size_t)(k + 1) * N + 0];
+        hzo_lu_solve(LU, npivot, ul + 1, n);
+    }
+    if (ur) {
+        ur[0] = 0.0;
+        ur[N - 1] = 1.0;
+        for (int k = 0; k < n; ++k) ur[1 + k] = -Dzz[(size_t)(k + 1) * N + (N - 1)];
+        hzo_lu_solve(LU, npivot, ur + 1, n);
+    }
+    return HZO_OK;
 }

@@ -88,8 +88,18 @@ int hzo_solve(const hzo_desc *d, const double *f, const double *gl,
               const double *gr, double *u);
 
 /* FirstDerivativeZ / SecondDerivativeZ, NonPeriodic3D_MPI.cpp:632-683, on
- * the global layout (element-wise application of the scaled matrices). */
+ * the reference's element arrays: u, out [size*Ns][Nx][Ny][N] (element e of
+ * rank r at e = r*Ns + kk, interface nodes stored in both neighbours),
+ * out = Dz u (order 1) or Dzz u (order 2) per element column, summed over
+ * p = 0..N-1 in order. */
 int hzo_derivative_z(const hzo_desc *d, int order, const double *u, double *out);
+/* ConstructFundamentalSolutions, NonPeriodic3D_MPI.cpp:184-232, for one
+ * column: U[kk][0..N) (kk < Ns) = 0 at the ends and the interior solve of
+ * rhs[kk][1..N-2] with the factored operator (LU, npivot from
+ * hzo_lu_decompose, order N-2); ul, ur (N each, may be NULL) = the
+ * homogeneous solutions with -Dzz[1..N-2][0] and -Dzz[1..N-2][N-1]. */
+int hzo_fundamental_solutions(int N, int Ns, const double *LU, const int *npivot, const double *Dzz,
+                              const double *rhs, double *U, double *ul, double *ur);
 /* Periodic3D_MPI.cpp SolvePDE at size = 1: u[Nx][Ny][Ktot] from f (z periodic). */
 int hzo_solve_periodic(const hzo_desc *d, const double *f, double *u);
 /* The same, also writing alphaPDE[Nx][2 Ny] (the closure coefficient per

@@ -70,6 +70,7 @@ def lib(fma=False):
         L.hzo_derivative_z.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, _D, _D]
         L.hzo_solve_periodic.argtypes = [ctypes.POINTER(Desc), _D, _D]
         L.hzo_solve_periodic_alpha.argtypes = [ctypes.POINTER(Desc), _D, _D, _D]
+        L.hzo_fundamental_solutions.argtypes = [ctypes.c_int, ctypes.c_int, _D, _I, _D, _D, _D, _D, _D]
         if fma:
             _fma = L
         else:
@@ -183,7 +184,26 @@ def solve_periodic(desc, f, fma=False, alpha=False):
 
 
 def derivative_z(desc, u, order):
+    """FirstDerivativeZ (order 1) / SecondDerivativeZ (order 2) of element
+    arrays u[size*Ns][Nx][Ny][N]."""
     u = np.ascontiguousarray(u, dtype=np.float64)
     out = np.zeros_like(u)
     _check(lib().hzo_derivative_z(ctypes.byref(desc), order, _p(u), _p(out)))
     return out
+
+
+def fundamental_solutions(N, LU, piv, Dzz, rhs):
+    """ConstructFundamentalSolutions for one column: rhs[Ns][N] (interior
+    entries used) -> (U[Ns][N], ul[N], ur[N]) with factors from lu()."""
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    Ns = rhs.shape[0]
+    LU = np.ascontiguousarray(LU, dtype=np.float64)
+    pv = np.zeros(N - 2, dtype=np.int32)
+    pv[: N - 3] = piv
+    Dzz = np.ascontiguousarray(Dzz, dtype=np.float64)
+    U = np.zeros((Ns, N))
+    ul = np.zeros(N)
+    ur = np.zeros(N)
+    _check(lib().hzo_fundamental_solutions(N, Ns, _p(LU), pv.ctypes.data_as(_I), _p(Dzz), _p(rhs), _p(U), _p(ul),
+                                           _p(ur)))
+    return U, ul, ur

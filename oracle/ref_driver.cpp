@@ -1,6 +1,8 @@
 // ref_driver.cpp -- fixture driver linked against gpritam/MPI's
 // DirectPseudoSpectral sources, compiled where they lie under the reference
-// tree by oracle/ref.mk into oracle/_ref/ (never committed, never shipped).
+// tree by oracle/Makefile into oracle/_ref/ (never committed).  bench.py's
+// CPU baseline runs it on the GPU machine's host cores, which is why the
+// built binary travels with the repository snapshot there.
 //
 // TEST INFRASTRUCTURE ONLY: it writes the golden vectors under tests/golden/
 // that pin the oracle (hz_oracle.c).  It contains no reference code; it only
@@ -19,6 +21,21 @@
 //                                   (run under mpiexec -np P) IN: f[Nx][Ny][Ktot]
 //                                   gl[Nx][Ny] gr[Nx][Ny]; OUT.rankR: this
 //                                   rank's [Ns][Nx][Ny][N] block, then z.
+//   zderiv Nx Ny Nz N zl zr IN OUT  (under mpiexec -np P) IN: element arrays
+//                                   u[P*Ns][Nx][Ny][N]; OUT.rankR: this rank's
+//                                   FirstDerivativeZ then SecondDerivativeZ
+//                                   blocks [Ns][Nx][Ny][N]
+//   fundsol Nx Ny Nz N xl xr yl yr zl zr lambda iw jw i j IN OUT
+//                                   operator of mode (iw,jw) factored by the
+//                                   test callback, rhsPDE[kk][i][j][0..N) from
+//                                   IN[Ns][N], then ConstructFundamentalSolutions
+//                                   (U, i, j); OUT: U[kk].phi[i][j][0..N)
+//   lusolve n COUNT IN OUT          IN per system: A[n*n] b[n]; OUT per system:
+//                                   LUSolve's x[n] after LUDecomposition
+//   write Nx Ny Nz N xl xr yl yr zl zr IN
+//                                   U[Ns][Nx][Ny][N] from IN, then WriteFile
+//                                   with tecplot_exact (below) into
+//                                   ./Output/Solution-0000.tec
 #include "NonPeriodic3D_MPI.h"
 
 #include <cstdio>
@@ -215,6 +232,133 @@ static int mode_fderiv(int argc, char **argv)
     return 0;
 }
 
+static int mode_zderiv(int argc, char **argv)
+{
+    if (argc < 10) return 2;
+    MPI_Init(&argc, &argv);
+    MPI_Comm_rank(MPI_COMM_WORLD, &Rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    const int nx = std::atoi(argv[2]), ny = std::atoi(argv[3]), nz = std::atoi(argv[4]), np = std::atoi(argv[5]);
+    CreatePDEPlan(nx, ny, nz, np, 0.0, 1.0, 0.0, 1.0, std::atof(argv[6]), std::atof(argv[7]));
+    const size_t col = (size_t)Nx * Ny * N;
+    std::vector<double> in = read_doubles(argv[8], (size_t)size * Ns * col);
+    FieldVariable *U = new FieldVariable[Ns], *Uz = new FieldVariable[Ns], *Uzz = new FieldVariable[Ns];
+    for (int kk = 0; kk < Ns; ++kk) {
+        Allocate(U[kk].phi, 2 * Nx, NY, N);
+        Allocate(Uz[kk].phi, 2 * Nx, NY, N);
+        Allocate(Uzz[kk].phi, 2 * Nx, NY, N);
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k)
+                    U[kk].phi[i][j][k] = in[(size_t)(Rank * Ns + kk) * col + ((size_t)i * Ny + j) * N + k];
+    }
+    FirstDerivativeZ(Uz, U);
+    SecondDerivativeZ(Uzz, U);
+    std::string path = std::string(argv[9]) + ".rank" + std::to_string(Rank);
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    for (FieldVariable *F : {Uz, Uzz})
+        for (int kk = 0; kk < Ns; ++kk)
+            for (int i = 0; i < Nx; ++i)
+                for (int j = 0; j < Ny; ++j) write_raw(fp, F[kk].phi[i][j], sizeof(double) * N);
+    std::fclose(fp);
+    for (int kk = 0; kk < Ns; ++kk) {
+        Deallocate(U[kk].phi, 2 * Nx, NY, N);
+        Deallocate(Uz[kk].phi, 2 * Nx, NY, N);
+        Deallocate(Uzz[kk].phi, 2 * Nx, NY, N);
+    }
+    delete[] U;
+    delete[] Uz;
+    delete[] Uzz;
+    DestroyPDEPlan();
+    MPI_Finalize();
+    return 0;
+}
+
+static int mode_fundsol(int argc, char **argv)
+{
+    if (argc < 19) return 2;
+    Rank = 0;
+    size = 1;
+    double b[6];
+    for (int i = 0; i < 6; ++i) b[i] = std::atof(argv[6 + i]);
+    CreatePDEPlan(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), b[0], b[1], b[2],
+                  b[3], b[4], b[5]);
+    Lambda = std::atof(argv[12]);
+    const int iw = std::atoi(argv[13]), jw = std::atoi(argv[14]), i = std::atoi(argv[15]), j = std::atoi(argv[16]);
+    std::vector<double> in = read_doubles(argv[17], (size_t)Ns * N);
+    FieldVariable *U = new FieldVariable[Ns];
+    for (int kk = 0; kk < Ns; ++kk) {
+        Allocate(U[kk].phi, 2 * Nx, NY, N);
+        for (int k = 0; k < N; ++k) rhsPDE[kk][i][j][k] = in[(size_t)kk * N + k];
+    }
+    construct_operator(iw, jw);
+    ConstructFundamentalSolutions(U, i, j);
+    FILE *fp = std::fopen(argv[18], "wb");
+    for (int kk = 0; kk < Ns; ++kk) write_raw(fp, U[kk].phi[i][j], sizeof(double) * N);
+    std::fclose(fp);
+    for (int kk = 0; kk < Ns; ++kk) Deallocate(U[kk].phi, 2 * Nx, NY, N);
+    delete[] U;
+    DestroyPDEPlan();
+    return 0;
+}
+
+static int mode_lusolve(int argc, char **argv)
+{
+    if (argc < 6) return 2;
+    const int n = std::atoi(argv[2]), count = std::atoi(argv[3]);
+    const size_t per = (size_t)n * n + n;
+    std::vector<double> in = read_doubles(argv[4], (size_t)count * per);
+    FILE *fp = std::fopen(argv[5], "wb");
+    double **A, *x;
+    int *piv;
+    Allocate(A, n, n);
+    Allocate(x, n);
+    Allocate(piv, n);
+    for (int c = 0; c < count; ++c) {
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) A[i][j] = in[c * per + (size_t)i * n + j];
+        for (int i = 0; i < n; ++i) x[i] = in[c * per + (size_t)n * n + i];
+        double det = 0.0;
+        LUDecomposition(A, piv, det, n);
+        LUSolve(A, piv, x, n);
+        write_raw(fp, x, sizeof(double) * n);
+    }
+    std::fclose(fp);
+    return 0;
+}
+
+// The exact function handed to WriteFile: a polynomial, so that the
+// reference build and a drop-in caller evaluate it to the same bits.
+static double tecplot_exact(const double x, const double y, const double zz)
+{
+    return x + 2.0 * y - 3.0 * zz + x * y * zz;
+}
+
+static int mode_write(int argc, char **argv)
+{
+    if (argc < 13) return 2;
+    Rank = 0;
+    size = 1;
+    double b[6];
+    for (int i = 0; i < 6; ++i) b[i] = std::atof(argv[6 + i]);
+    CreatePDEPlan(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), b[0], b[1], b[2],
+                  b[3], b[4], b[5]);
+    const size_t col = (size_t)Nx * Ny * N;
+    std::vector<double> in = read_doubles(argv[12], (size_t)Ns * col);
+    FieldVariable *U = new FieldVariable[Ns];
+    for (int kk = 0; kk < Ns; ++kk) {
+        Allocate(U[kk].phi, 2 * Nx, NY, N);
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j)
+                for (int k = 0; k < N; ++k) U[kk].phi[i][j][k] = in[(size_t)kk * col + ((size_t)i * Ny + j) * N + k];
+    }
+    WriteFile(U, tecplot_exact);
+    for (int kk = 0; kk < Ns; ++kk) Deallocate(U[kk].phi, 2 * Nx, NY, N);
+    delete[] U;
+    DestroyPDEPlan();
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
@@ -224,6 +368,10 @@ int main(int argc, char **argv)
     if (mode == "fft") return mode_fft(argc, argv);
     if (mode == "solve") return mode_solve(argc, argv);
     if (mode == "fderiv") return mode_fderiv(argc, argv);
+    if (mode == "zderiv") return mode_zderiv(argc, argv);
+    if (mode == "fundsol") return mode_fundsol(argc, argv);
+    if (mode == "lusolve") return mode_lusolve(argc, argv);
+    if (mode == "write") return mode_write(argc, argv);
     std::fprintf(stderr, "ref_driver: unknown mode %s\n", mode.c_str());
     return 2;
 }

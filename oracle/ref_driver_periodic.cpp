@@ -1,7 +1,7 @@
 // ref_driver_periodic.cpp -- fixture driver for gpritam/MPI's periodic-z
 // DirectPseudoSpectral solver (HelmholtzSolvers/DirectPseudoSpectral/src/
 // Periodic3D_MPI.cpp), compiled where the reference sources lie by
-// oracle/Makefile into oracle/_ref/ (never committed, never shipped).
+// oracle/Makefile into oracle/_ref/ (never committed).
 //
 // TEST INFRASTRUCTURE ONLY: it writes golden vectors under tests/golden/.
 // It contains no reference code; it only calls the reference's public
@@ -10,7 +10,9 @@
 //
 //   psolve Nx Ny Nz N xl xr yl yr zl zr lambda IN OUT
 //       (under mpiexec -np P) IN: f[Nx][Ny][Ktot]; OUT.rankR: this rank's
-//       [Ns][Nx][Ny][N] block, then z.
+//       [Ns][Nx][Ny][N] block, then z; OUT.alpha (rank 0): the closure
+//       coefficients alphaPDE[Nx][2 Ny] after the solve (broadcast to every
+//       rank by the reference; zeros when the run has one element).
 #include "Periodic3D_MPI.h"
 
 #include <cstdio>
@@ -80,7 +82,12 @@ int main(int argc, char **argv)
             for (int j = 0; j < Ny; ++j) write_raw(fp, V[kk].phi[i][j], sizeof(double) * N);
     write_raw(fp, z, sizeof(double) * (Ns * (N - 1) + 1));
     std::fclose(fp);
-    if (Rank == 0) std::printf("solve_seconds %.6f Ns %d size %d\n", t1 - t0, Ns, size);
+    if (Rank == 0) {
+        FILE *fa = std::fopen((std::string(argv[14]) + ".alpha").c_str(), "wb");
+        for (int i = 0; i < Nx; ++i) write_raw(fa, alphaPDE[i], sizeof(double) * NY);
+        std::fclose(fa);
+        std::printf("solve_seconds %.6f Ns %d size %d\n", t1 - t0, Ns, size);
+    }
     for (int kk = 0; kk < Ns; ++kk) Deallocate(V[kk].phi, 2 * Nx, NY, N);
     delete[] V;
     DestroyFFTPlan(Maximum(Nx, Ny));

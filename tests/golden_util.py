@@ -164,3 +164,51 @@ def lowdim_sample(u, rec):
         return np.asarray(u).ravel(), ref.ravel()
     idx = rec["sample_idx"]
     return np.array([u[i, k] for i, k in idx]), unhex(rec["sample_u"])
+
+
+# ---------------------------------------------------------------------------
+# Helper fixtures: zderiv.json, fundsol.json, lusolve.json, tecplot.json,
+# palpha.json (tests/golden/make_golden.py --helpers / --palpha).  Their
+# inputs are seeded uniform fields.
+
+def zderiv_inputs(rec):
+    """Element arrays u[E][Nx][Ny][N] of a zderiv.json case."""
+    from mpi_amd import workloads as wl
+    u = wl.uniform_field((rec["elements"], rec["Nx"], rec["Ny"], rec["N"]), -1.0, 1.0, rec["seed"])
+    assert sha(u) == rec["sha_u"], "input generator drifted from the golden vectors"
+    return u
+
+
+def fundsol_inputs(rec):
+    """rhsPDE[Ns][N] of the column of a fundsol.json case."""
+    from mpi_amd import workloads as wl
+    rhs = wl.uniform_field((rec["Ns"], rec["N"]), -1.0, 1.0, rec["seed"])
+    assert sha(rhs) == rec["sha_rhs"], "input generator drifted from the golden vectors"
+    return rhs
+
+
+def lusolve_inputs(name, rec):
+    """(A, b) of a lusolve.json case."""
+    import tests_lu_cases
+    from mpi_amd import workloads as wl
+    A = dict((nm, a) for nm, _, a in tests_lu_cases.lu_cases())[name]
+    b = wl.uniform_field((rec["n"],), -1.0, 1.0, rec["seed"])
+    assert sha(A) == rec["sha_input"] and sha(b) == rec["sha_b"], "input generator drifted"
+    return A, b
+
+
+def tecplot_inputs(rec):
+    """Element arrays U[Ns][Nx][Ny][N] written by a tecplot.json case."""
+    from mpi_amd import workloads as wl
+    nx, ny, _, N = (int(v) for v in rec["args"][:4])
+    U = wl.uniform_field((rec["Ns"], nx, ny, N), -2.0, 2.0, rec["seed"])
+    assert sha(U) == rec["sha_u"], "input generator drifted from the golden vectors"
+    return U
+
+
+def palpha_inputs(rec):
+    """f[Nx][Ny][K] of a palpha.json case."""
+    from mpi_amd import workloads as wl
+    f = wl.uniform_field((rec["Nx"], rec["Ny"], rec["K"]), -1.0, 1.0, rec["seed"])
+    assert sha(f) == rec["sha_f"], "input generator drifted from the golden vectors"
+    return f

@@ -402,3 +402,18 @@ def test_cubic_rhs_matches_torch(cuda, n):
     assert torch.equal(v, f)
     with pytest.raises(_lib.HzError):
         _lib.check(lib.hz_cubic_rhs(None, f.data_ptr(), 5, a3, a1, s))
+
+
+def test_allen_cahn_graph_refuses_stale_eps(cuda):
+    """A captured forward holds its eps in the right-hand side's
+    coefficients: after the plan's lambda changes, replay() refuses."""
+    import torch
+    from mpi_amd import workloads as wl
+    from mpi_amd.allen_cahn import AllenCahn
+    ac = AllenCahn(nx=16, ny=16, n=9, steps=2, checkpoint_every=1)
+    u0 = torch.from_numpy(wl.uniform_field((16, 16, ac.K), -0.05, 0.05, 4)).cuda()
+    replay, u_in, u_out = ac.capture_run(u0, 0.02)
+    replay()
+    ac.run(u0, 0.03)  # sets the plan's lambda for eps = 0.03
+    with pytest.raises(RuntimeError):
+        replay()

@@ -42,6 +42,15 @@ def test_callback_with_general_operator(cuda):
     assert err <= 2.0e-13
 
 
+def test_dropin_diag_form(cuda):
+    """SolverForm = HZ_FORM_DIAG: the standard operator is recognised and the
+    z-diagonalised solve gives the reference's error on the shipped problem."""
+    r = run("diag")
+    assert r.returncode == 0, r.stdout + r.stderr
+    err, _ = errors(r.stdout)
+    assert err <= 2.0e-13
+
+
 def test_singular_operator_exits_like_reference(cuda):
     r = run("singular")
     assert r.returncode == 1

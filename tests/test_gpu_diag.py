@@ -1,7 +1,10 @@
 """DIAG form (z-diagonalised, FP64 MFMA) against the oracle and the reference.
 
-Bar: relative max-norm error <= 1e-10 against the FP64 oracle (the chosen-form
-bound); the measured errors are ~1e-13.
+Bar: relative max-norm error <= 1e-12 against the FP64 oracle and the
+reference's own output (measured ~6e-14 at 256 x 256 x 129).  On rough random
+inputs the bar is max(1e-12, 2 x the rounding spread of the reference
+algorithm itself: its restatement built with fused multiply-adds against the
+plain build), the bar of the low-dimensional tests.
 """
 import math
 
@@ -12,7 +15,15 @@ from golden_util import case_inputs, load, sample
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-10
+TOL = 1e-12
+
+
+def spread_bar(desc, f, gl, gr, uo):
+    """max(TOL, 2 x how far rounding alone moves the reference algorithm on
+    this input: the oracle built with fused multiply-adds against the
+    oracle)."""
+    uf = po.solve(desc, f, gl, gr, fma=True)
+    return max(TOL, 2.0 * float(np.abs(uf - uo).max() / np.abs(uo).max()))
 
 
 def to_dev(a):
@@ -46,6 +57,7 @@ def test_diag_matches_oracle(cuda, name):
     rec, desc, f, gl, gr, ex, u = solve_case(name)
     desc.size = 1
     uo = po.solve(desc, f, gl, gr)
+    print(f"{name}: rel err {rel_err(u, uo):.2e}")
     assert rel_err(u, uo) <= TOL
     if ex is not None:
         assert np.abs(u - ex).max() <= 10 * rec["err_vs_exact"] + 1e-13
@@ -54,6 +66,7 @@ def test_diag_matches_oracle(cuda, name):
 def test_diag_config2_matches_reference(cuda):
     rec, desc, f, gl, gr, ex, u = solve_case("config2_N129_np1")
     got, want = sample(u, rec)
+    print(f"config2 diag: rel err vs the reference {np.abs(got - want).max() / rec['max_abs_u']:.2e}")
     assert np.abs(got - want).max() <= TOL * rec["max_abs_u"]
 
 
@@ -81,7 +94,9 @@ def test_diag_general_against_oracle(cuda, N, ne, robin):
     gr = rng.standard_normal((nx, ny))
     u = plan.solve(to_dev(f), to_dev(gl), to_dev(gr)).cpu().numpy()
     uo = po.solve(desc, f, gl, gr)
-    assert rel_err(u, uo) <= TOL
+    err, bar = rel_err(u, uo), spread_bar(desc, f, gl, gr, uo)
+    print(f"N={N} ne={ne} robin={robin}: rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
 
 
 @pytest.mark.parametrize("N", [33, 129, 257])
@@ -102,7 +117,10 @@ def test_diag_plan_autotune(cuda, N):
     f = rng.standard_normal((64, 16, plan.nz))
     g = rng.standard_normal((64, 16))
     u = plan.solve(to_dev(f), to_dev(g), to_dev(g)).cpu().numpy()
-    assert rel_err(u, po.solve(desc, f, g, g)) <= TOL
+    uo = po.solve(desc, f, g, g)
+    err, bar = rel_err(u, uo), spread_bar(desc, f, g, g, uo)
+    print(f"autotuned N={N}: rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
     plan.close()
 
 
@@ -135,7 +153,10 @@ def test_diag_every_tile_shape(cuda, N, ne, robin):
         assert torch.equal(u, outs[0]), f"tile {rt} differs from tile {shapes[0]}"
     desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
                         alphal=al, betal=bl, alphar=ar, betar=br, lam=3.0)
-    assert rel_err(outs[0].cpu().numpy(), po.solve(desc, f, gl, gr)) <= TOL
+    uo = po.solve(desc, f, gl, gr)
+    err, bar = rel_err(outs[0].cpu().numpy(), uo), spread_bar(desc, f, gl, gr, uo)
+    print(f"tiles N={N} ne={ne}: rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
     plan.close()
 
 
@@ -189,5 +210,8 @@ def test_largest_transform_lengths_solve(cuda, nx, ny, form):
     u = plan.solve(to_dev(f), to_dev(gl), to_dev(gr)).cpu().numpy()
     desc = po.make_desc(nx, ny, N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=2 * math.pi, zl=-1, zr=1,
                         alphal=0.3, betal=1.0, alphar=0.2, betar=1.0, lam=2.0)
-    assert rel_err(u, po.solve(desc, f, gl, gr)) <= (1e-12 if form == "plain" else TOL)
+    uo = po.solve(desc, f, gl, gr)
+    err, bar = rel_err(u, uo), spread_bar(desc, f, gl, gr, uo)
+    print(f"{nx}x{ny} {form}: rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
     plan.close()

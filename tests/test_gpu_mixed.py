@@ -1,8 +1,9 @@
 """MIXED form: FP32 solution operators with FP64 residual refinement.
 
-Bar: relative max-norm error <= 1e-10 against the FP64 oracle / the
-reference's own output, with the error and the residual history stated per
-refinement sweep.
+Bar: relative max-norm error <= 1e-12 against the FP64 oracle / the
+reference's own output (max(1e-12, 2 x the reference algorithm's own rounding
+spread) on rough random inputs), with the error and the residual history
+stated per refinement sweep.
 """
 import math
 import os
@@ -14,7 +15,15 @@ from golden_util import GOLDEN, case_inputs, load, sample
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-10
+TOL = 1e-12
+
+
+def spread_bar(desc, f, gl, gr, uo):
+    """max(TOL, 2 x how far rounding alone moves the reference algorithm on
+    this input: the oracle built with fused multiply-adds against the
+    oracle)."""
+    uf = po.solve(desc, f, gl, gr, fma=True)
+    return max(TOL, 2.0 * float(np.abs(uf - uo).max() / np.abs(uo).max()))
 
 
 def dev(a):
@@ -70,9 +79,10 @@ def test_mixed_robin_against_oracle(cuda, N, ne):
     gl = rng.standard_normal((nx, ny))
     gr = rng.standard_normal((nx, ny))
     u = plan.solve(dev(f), dev(gl), dev(gr)).cpu().numpy()
-    err = rel_err(u, po.solve(desc, f, gl, gr))
-    print(f"N={N} ne={ne}: rel err {err:.3e}, fp64 modes {plan.mixed_fp64_modes()}/{plan.num_modes}")
-    assert err <= TOL
+    uo = po.solve(desc, f, gl, gr)
+    err, bar = rel_err(u, uo), spread_bar(desc, f, gl, gr, uo)
+    print(f"N={N} ne={ne}: rel err {err:.3e}, bar {bar:.2e}, fp64 modes {plan.mixed_fp64_modes()}/{plan.num_modes}")
+    assert err <= bar
     plan.close()
 
 

@@ -1,7 +1,7 @@
 """Periodic-z solver (Periodic3D_MPI.cpp) on the GPU, PLAIN_LU, DIAG and MIXED
 forms, against the reference's own output (golden vectors) and the oracle.
 
-Bars: PLAIN <= 1e-12, DIAG and MIXED (three refinement sweeps) <= 1e-10
+Bars: PLAIN, DIAG and MIXED (three refinement sweeps) <= 1e-12
 relative max-norm error.
 """
 import numpy as np
@@ -11,7 +11,7 @@ from golden_util import load, periodic_case_inputs, sample
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
-TOL = {"plain": 1e-12, "diag": 1e-10, "mixed": 1e-10}
+TOL = {"plain": 1e-12, "diag": 1e-12, "mixed": 1e-12}
 
 
 def dev(a):
@@ -105,7 +105,7 @@ def test_periodic_closure_matches_oracle(cuda, form):
             full[i, 2 * m + 1] = half[1, ii, mm] if low else -half[1, ii, mm]
     _, want = po.solve_periodic(desc, f, alpha=True)
     assert np.abs(want).max() > 0
-    assert np.abs(full - want).max() <= 1e-10 * np.abs(want).max()
+    assert np.abs(full - want).max() <= 1e-12 * np.abs(want).max()
     one = HelmholtzPlan(8, 8, 9, 1, lam=1.0, form=form, periodic_z=True)
     one.solve(dev(np.ones((8, 8, one.nz))))
     assert not one.periodic_closure().any()  # one element: no closure, as in the reference
