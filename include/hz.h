@@ -124,7 +124,8 @@ int hz_solve(hz_plan *plan, const double *f, const double *g_l, const double *g_
  *  op_diag != NULL, op_full == NULL : operator m = Dzz_int with its diagonal
  *     replaced by op_diag[m*(n-2) .. +n-2)  (device, float64)
  *  op_full != NULL : operator m = op_full[m*(n-2)^2 ..] row-major (device).
- * PLAIN_LU form only. */
+ * Runs the reference-algorithm (PLAIN_LU) kernel whatever the plan's form;
+ * needs N - 2 <= 128. */
 int hz_solve_operators(hz_plan *plan, const double *f, const double *g_l,
                        const double *g_r, const double *op_diag,
                        const double *op_full, double *u, hz_stream_t stream);
@@ -144,6 +145,20 @@ int hz_solve_status(hz_plan *plan, hz_stream_t stream);
  * there is no face data and glbar, grbar must be NULL. */
 int hz_adjoint(hz_plan *plan, const double *ubar, const double *u, double *fbar,
                double *glbar, double *grbar, double *lambda_bar, hz_stream_t stream);
+
+/* No counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, serving loops that forbid hipMalloc): hz_adjoint
+ * that allocates nothing and does not synchronise.  dL/dlambda, when
+ * lambda_bar_dev != NULL (then u is required), is written to that DEVICE
+ * word on `stream`; the reduction's partial sums live in a static device
+ * array of 1024 doubles (8 KiB per GPU, kept until the process ends), enough
+ * for any field size.  The plan's diagonalised operators must exist: DIAG and
+ * MIXED plans have them from creation, a PLAIN_LU plan after its first
+ * hz_adjoint call (HZ_ERR_UNSUPPORTED before).  The plan's device must be
+ * current.  Calls that overlap on one GPU (two streams at once) race on the
+ * shared array.  Other arguments and results as hz_adjoint. */
+int hz_adjoint_noalloc(hz_plan *plan, const double *ubar, const double *u, double *fbar, double *glbar,
+                       double *grbar, double *lambda_bar_dev, hz_stream_t stream);
 
 /* No counterpart in the reference: the explicit right-hand side of a
  * semi-implicit Allen-Cahn step, f[i] = a3 u[i]^3 + a1 u[i] for i < n

@@ -52,6 +52,7 @@ SIGNATURES = {
     "hz_solve_operators": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "hz_solve_status": (ctypes.c_int, [_VP, _VP]),
     "hz_adjoint": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _PD, _VP]),
+    "hz_adjoint_noalloc": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "hz_derivative_z": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP]),
     "hz_derivative_xy": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_int, _VP]),
     "hz_fourier_xy": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, ctypes.c_int, _VP]),

@@ -156,11 +156,19 @@ int launch_derivative(const double *D, int N, long long ncols, const double *u, 
 int launch_spectral_derivative(double *re, double *im, long long A, long long B, long long lanes, int axis_a, int L,
                                double len, int order, hipStream_t s);
 // sum_i a[i]*b[i] into *out (device or pinned host memory), deterministic two-pass reduction;
-// partial >= 1024 doubles
+// partial >= kDotPartials doubles (the first pass runs at most that many work-groups)
+constexpr int kDotPartials = 1024;
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
 // n pseudo-random doubles in [-0.5, 0.5) (a hash of the index and seed):
 // timing data for the plan-time tuners.
 int launch_fill_noise(double *p, size_t n, unsigned long long seed, hipStream_t s);
+
+// hz_adjoint on a plan whose diagonalised operators exist, with dL/dlambda
+// (when lambda_bar_dev is not null) written to device memory through the
+// caller's reduction scratch `partial` (kDotPartials doubles): allocates
+// nothing and does not synchronise.
+int adjoint_prebuilt(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar, double *grbar,
+                     double *lambda_bar_dev, double *partial, hipStream_t s);
 
 }  // namespace hz
 

@@ -117,7 +117,7 @@ int launch_spectral_derivative(double *re, double *im, long long A, long long B,
 
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s)
 {
-    const int blocks = (int)std::min<long long>(1024, std::max<long long>(1, (n + 255) / 256));
+    const int blocks = (int)std::min<long long>(kDotPartials, std::max<long long>(1, (n + 255) / 256));
     hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(256), 0, s, a, b, n, partial);
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, partial, blocks, out);
     HZ_HIP(hipGetLastError());

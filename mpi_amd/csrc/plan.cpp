@@ -868,21 +868,12 @@ int hz_solve_status(hz_plan *p, hz_stream_t stream)
 // (diag.hip, ADJ) in the middle.  A PLAIN_LU plan builds the diagonalised
 // operators on its first adjoint call; both forms solve the same discrete
 // system, so they share one adjoint.
-int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar,
-               double *grbar, double *lambda_bar, hz_stream_t stream)
+static int adjoint_pipeline(hz_plan *p, const double *ubar, double *fbar, double *glbar, double *grbar,
+                            hipStream_t s, StageRecorder *rec)
 {
-    if (!p || !ubar || !fbar) return set_error(HZ_ERR_ARG, "null argument");
-    if (p->d.periodic_z ? (glbar || grbar) : (!glbar || !grbar))
-        return set_error(HZ_ERR_ARG, "face gradients: required without periodic z, NULL with it");
-    if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
-    const hipStream_t s = (hipStream_t)stream;
     int st;
-    if (!p->d_Vrow) {
-        if ((st = setup_diag(p)) || (st = compute_homogeneous(p))) return st;
-    }
-    StageRecorder rec(p, s);
     if ((st = forward_xy(p, ubar, p->spec.re, p->spec.im, p->K, s, nullptr, nullptr, p->xs))) return st;
-    rec.mark("fft_forward");
+    if (rec) rec->mark("fft_forward");
     DiagArgs a = diag_args(p);
     a.adjoint = 1;
     a.VinvT = p->d_Vrow;
@@ -890,11 +881,54 @@ int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, do
     a.gbar_re = p->bc.re;
     a.gbar_im = p->bc.im;
     if ((st = launch_diag_modes(a, s))) return st;
-    rec.mark("diag_adjoint");
+    if (rec) rec->mark("diag_adjoint");
     if ((st = inverse_xy(p, p->spec.re, p->spec.im, fbar, p->K, s, glbar, grbar, p->xs))) return st;
-    rec.mark("fft_inverse");
+    if (rec) rec->mark("fft_inverse");
+    return HZ_OK;
+}
+
+static int adjoint_args(const hz_plan *p, const double *ubar, const double *u, const double *fbar,
+                        const double *glbar, const double *grbar, const void *lambda_bar)
+{
+    if (!p || !ubar || !fbar) return set_error(HZ_ERR_ARG, "null argument");
+    if (p->d.periodic_z ? (glbar || grbar) : (!glbar || !grbar))
+        return set_error(HZ_ERR_ARG, "face gradients: required without periodic z, NULL with it");
+    if (lambda_bar && !u) return set_error(HZ_ERR_ARG, "lambda_bar needs the forward solution u");
+    return HZ_OK;
+}
+
+}  // extern "C"
+
+namespace hz {
+int adjoint_prebuilt(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar, double *grbar,
+                     double *lambda_bar_dev, double *partial, hipStream_t s)
+{
+    int st;
+    if ((st = adjoint_args(p, ubar, u, fbar, glbar, grbar, lambda_bar_dev))) return st;
+    if (!p->d_Vrow)
+        return set_error(HZ_ERR_UNSUPPORTED, "the plan's diagonalised operators are not built yet (a PLAIN_LU plan "
+                                             "builds them on its first hz_adjoint call)");
+    if ((st = adjoint_pipeline(p, ubar, fbar, glbar, grbar, s, nullptr))) return st;
+    if (lambda_bar_dev) return launch_dot(fbar, u, (long long)p->nx * p->ny * p->K, partial, lambda_bar_dev, s);
+    return HZ_OK;
+}
+}  // namespace hz
+
+extern "C" {
+
+int hz_adjoint(hz_plan *p, const double *ubar, const double *u, double *fbar, double *glbar,
+               double *grbar, double *lambda_bar, hz_stream_t stream)
+{
+    int st;
+    if ((st = adjoint_args(p, ubar, u, fbar, glbar, grbar, lambda_bar))) return st;
+    const hipStream_t s = (hipStream_t)stream;
+    if (!p->d_Vrow) {
+        if ((st = setup_diag(p)) || (st = compute_homogeneous(p))) return st;
+    }
+    StageRecorder rec(p, s);
+    if ((st = adjoint_pipeline(p, ubar, fbar, glbar, grbar, s, &rec))) return st;
     if (lambda_bar) {
-        if (!p->d_partial && (st = dmalloc(&p->d_partial, 1024))) return st;
+        if (!p->d_partial && (st = dmalloc(&p->d_partial, kDotPartials))) return st;
         // the final reduction writes straight into pinned host memory (no
         // separate device-to-host copy before the wait)
         if (!p->h_lambda_bar && hipHostMalloc((void **)&p->h_lambda_bar, sizeof(double), hipHostMallocCoherent) != hipSuccess) {
