@@ -17,10 +17,14 @@
 //   fderiv Nx Ny Nz Lx Ly IN OUT    FirstDerivativeFourierX/Y, SecondDerivative-
 //                                   FourierX/Y on F[Nx][Ny][Nz]; OUT: the four
 //                                   results in that order
-//   solve Nx Ny Nz N xl xr yl yr zl zr al bl ar br lambda IN OUT
+//   solve Nx Ny Nz N xl xr yl yr zl zr al bl ar br lambda IN OUT [REPS]
 //                                   (run under mpiexec -np P) IN: f[Nx][Ny][Ktot]
 //                                   gl[Nx][Ny] gr[Nx][Ny]; OUT.rankR: this
 //                                   rank's [Ns][Nx][Ny][N] block, then z.
+//                                   REPS solves (default 1), the inputs
+//                                   refilled before each (SolvePDE destroys
+//                                   them); rank 0 prints every solve's
+//                                   barrier-to-barrier seconds.
 //   zderiv Nx Ny Nz N zl zr IN OUT  (under mpiexec -np P) IN: element arrays
 //                                   u[P*Ns][Nx][Ny][N]; OUT.rankR: this rank's
 //                                   FirstDerivativeZ then SecondDerivativeZ
@@ -171,21 +175,25 @@ static int mode_solve(int argc, char **argv)
     std::vector<double> in = read_doubles(argv[17], nf + 2 * np2);
     FieldVariable *V = new FieldVariable[Ns];
     for (int kk = 0; kk < Ns; ++kk) Allocate(V[kk].phi, 2 * Nx, NY, N);
-    for (int kk = 0; kk < Ns; ++kk)
+    const int reps = argc > 19 ? std::atoi(argv[19]) : 1;
+    std::vector<double> secs;
+    for (int rep = 0; rep < (reps > 0 ? reps : 1); ++rep) {
+        for (int kk = 0; kk < Ns; ++kk)
+            for (int i = 0; i < Nx; ++i)
+                for (int j = 0; j < Ny; ++j)
+                    for (int k = 0; k < N; ++k)
+                        rhsPDE[kk][i][j][k] = in[((size_t)i * Ny + j) * ktot + (size_t)(Rank * Ns + kk) * (N - 1) + k];
         for (int i = 0; i < Nx; ++i)
-            for (int j = 0; j < Ny; ++j)
-                for (int k = 0; k < N; ++k)
-                    rhsPDE[kk][i][j][k] = in[((size_t)i * Ny + j) * ktot + (size_t)(Rank * Ns + kk) * (N - 1) + k];
-    for (int i = 0; i < Nx; ++i)
-        for (int j = 0; j < Ny; ++j) {
-            leftBC[i][j] = in[nf + (size_t)i * Ny + j];
-            rightBC[i][j] = in[nf + np2 + (size_t)i * Ny + j];
-        }
-    MPI_Barrier(MPI_COMM_WORLD);
-    const double t0 = MPI_Wtime();
-    SolvePDE(V, construct_operator);
-    MPI_Barrier(MPI_COMM_WORLD);
-    const double t1 = MPI_Wtime();
+            for (int j = 0; j < Ny; ++j) {
+                leftBC[i][j] = in[nf + (size_t)i * Ny + j];
+                rightBC[i][j] = in[nf + np2 + (size_t)i * Ny + j];
+            }
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double t0 = MPI_Wtime();
+        SolvePDE(V, construct_operator);
+        MPI_Barrier(MPI_COMM_WORLD);
+        secs.push_back(MPI_Wtime() - t0);
+    }
     std::string path = std::string(argv[18]) + ".rank" + std::to_string(Rank);
     FILE *fp = std::fopen(path.c_str(), "wb");
     for (int kk = 0; kk < Ns; ++kk)
@@ -193,7 +201,11 @@ static int mode_solve(int argc, char **argv)
             for (int j = 0; j < Ny; ++j) write_raw(fp, V[kk].phi[i][j], sizeof(double) * N);
     write_raw(fp, z, sizeof(double) * (Ns * (N - 1) + 1));
     std::fclose(fp);
-    if (Rank == 0) std::printf("solve_seconds %.6f Ns %d size %d\n", t1 - t0, Ns, size);
+    if (Rank == 0) {
+        std::printf("solve_seconds");
+        for (double t : secs) std::printf(" %.6f", t);
+        std::printf(" Ns %d size %d\n", Ns, size);
+    }
     for (int kk = 0; kk < Ns; ++kk) Deallocate(V[kk].phi, 2 * Nx, NY, N);
     delete[] V;
     DestroyFFTPlan(Maximum(Nx, Ny));
