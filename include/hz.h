@@ -59,10 +59,11 @@ enum {
  *  DIAG      : z-diagonalised form -- Dzz_int = V diag(mu) V^-1 is computed
  *              once per plan; a solve is two FP64 MFMA contractions per
  *              column batch plus an O(n) shift per column.  No factorisation.
- *  MIXED     : FP32 solution operator per (|kx|,|ky|) mode (FP64 for the
- *              ill-conditioned low-|k| modes), formed once per lambda; every
- *              solve applies it in FP32 and refines with FP64 residuals of the
- *              GLL operator (refine_sweeps sweeps).  Transforms stay FP64.
+ *  MIXED     : FP32 solves through the plan's eigenbasis (two FP32 matrix-core
+ *              contractions per column batch) refined with FP64 residuals of
+ *              the GLL operator (one FP64 matrix-core contraction per sweep,
+ *              refine_sweeps sweeps); no per-mode operators are stored.
+ *              Transforms stay FP64.
  */
 enum { HZ_FORM_PLAIN_LU = 0, HZ_FORM_DIAG = 1, HZ_FORM_MIXED = 2 };
 
@@ -79,11 +80,11 @@ typedef struct {
     double lambda;           /* Helmholtz shift (reference Lambda)          */
     double alpha_l, beta_l, alpha_r, beta_r; /* Robin coefficients          */
     int form;                /* HZ_FORM_*                                   */
-    int refine_sweeps;       /* MIXED only: refinement sweeps (0 = default 3,
+    int refine_sweeps;       /* MIXED only: refinement sweeps (0 = default 2,
                                 < 0 = none, <= 16)                          */
-    double mixed_fp64_cond;  /* MIXED only: shift sigma below which a mode keeps
-                                an FP64 operator (0 = built-in rule:
-                                cond * 2^-24 > 1)                        */
+    double mixed_fp64_cond;  /* not used (kept for the layout): the MIXED
+                                refinement converges for every shift, so no
+                                mode needs an FP64 operator               */
     int periodic_z;          /* 1: z periodic as in Periodic3D_MPI.cpp (u and
                                 u_z continuous across z = zr -> zl); the Robin
                                 coefficients and face data are not used.
@@ -239,10 +240,13 @@ int hz_solve_batched(hz_plan *const *plans, int count, const double *const *f, c
                      const double *const *gr, double *const *u, hz_stream_t stream);
 
 /* Refinement history of the last MIXED solve (synchronises the device):
- * rel_residual[s] = max over modes and columns of |b - A x|_max / |b|_max
- * after the FP32 solve (s = 0) and after each refinement sweep (s = 1..);
- * *sweeps = number of values written (refine_sweeps + 1, at most max_sweeps).
- * No counterpart in the reference (its LU is FP64 only). */
+ * rel_residual[s] = max over spectral columns and elements of
+ * |b - A x|_max / |b|_max of the iterate before refinement sweep s + 1,
+ * i.e. after the FP32 solve (s = 0) and after s sweeps (s = 1..S-1), with
+ * S = refine_sweeps (the residual after the last sweep is not formed: it
+ * would cost one more FP64 contraction); *sweeps = number of values written
+ * (S, at most max_sweeps).  No counterpart in the reference (its LU is FP64
+ * only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
 /* Plan-time autotuning results: tile rows of the diagonalised mode kernel
  * (DIAG plans; 0 otherwise) with the measured ms of each candidate (0 for
@@ -264,7 +268,8 @@ int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
  * alpha[plane][i * (ny/2 + 1) + m], plane 0 = real, 1 = imaginary part;
  * zeros with one element (the reference computes none).  Synchronises. */
 int hz_plan_periodic_closure(const hz_plan *plan, double *alpha);
-/* Number of modes of a MIXED plan that keep an FP64 operator. */
+/* Number of modes of a MIXED plan that keep an FP64 operator: 0 (every mode
+ * runs the FP32 eigenbasis solve; kept for callers of the earlier form). */
 int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
 
 /* Host-only utilities (no device needed): the plan-time setup that

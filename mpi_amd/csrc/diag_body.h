@@ -51,8 +51,47 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // acc[rb][jt] += A B over the k index, A = the LDS tile (rows
 // 16 (r0 + rb) + i, column k), B[k][j] = B[k * NP + j] from L2 (j in the
 // column tiles c0 .. c0 + JT - 1).  B is loaded two k-steps ahead so its L2
-// latency overlaps the MFMAs of the steps in between.
+// latency overlaps the MFMAs of the steps in between.  ROLLED keeps the k
+// loop rolled (kernels that run several contractions in a loop would
+// otherwise spill the hoisted B loads).
 template <int NP, int LD, int RBK, int JT>
+__device__ __forceinline__ void tile_gemm_steps(d4 (&acc)[RBK][JT], const double *tile, const double *bcol, int ks,
+                                                double (&b0)[JT], double (&b1)[JT], int li, int lk)
+{
+    constexpr int NK = NP / 4;
+    double n0[JT], n1[JT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) n0[jt] = n1[jt] = 0.0;
+    if (ks + 2 < NK) {
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) n0[jt] = bcol[(long long)4 * (ks + 2) * NP + 16 * jt];
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb) {
+        const double av = tile[(16 * rb + li) * LD + 4 * ks + lk];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+            acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0[jt], acc[rb][jt], 0, 0, 0);
+    }
+    if (ks + 3 < NK) {
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) n1[jt] = bcol[(long long)4 * (ks + 3) * NP + 16 * jt];
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBK; ++rb) {
+        const double av = tile[(16 * rb + li) * LD + 4 * (ks + 1) + lk];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+            acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1[jt], acc[rb][jt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+        b0[jt] = n0[jt];
+        b1[jt] = n1[jt];
+    }
+}
+
+template <int NP, int LD, int RBK, int JT, bool ROLLED = false>
 __device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile, const double *__restrict__ B,
                                           int c0, int r0, int li, int lk)
 {
@@ -66,37 +105,11 @@ __device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile
         b0[jt] = bcol[16 * jt];
         b1[jt] = bcol[(long long)4 * NP + 16 * jt];
     }
-    for (int ks = 0; ks < NK; ks += 2) {
-        double n0[JT], n1[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) n0[jt] = n1[jt] = 0.0;
-        if (ks + 2 < NK) {
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt) n0[jt] = bcol[(long long)4 * (ks + 2) * NP + 16 * jt];
-        }
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + 4 * ks + lk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0[jt], acc[rb][jt], 0, 0, 0);
-        }
-        if (ks + 3 < NK) {
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt) n1[jt] = bcol[(long long)4 * (ks + 3) * NP + 16 * jt];
-        }
-#pragma unroll
-        for (int rb = 0; rb < RBK; ++rb) {
-            const double av = tile[(16 * rb + li) * LD + 4 * (ks + 1) + lk];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[rb][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1[jt], acc[rb][jt], 0, 0, 0);
-        }
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) {
-            b0[jt] = n0[jt];
-            b1[jt] = n1[jt];
-        }
+    if constexpr (ROLLED) {
+#pragma nounroll
+        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT>(acc, tile, bcol, ks, b0, b1, li, lk);
+    } else {
+        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT>(acc, tile, bcol, ks, b0, b1, li, lk);
     }
 }
 

@@ -111,35 +111,35 @@ int diag_tile_choices(int NP, int *rt, int max);
 int launch_diag_modes(const DiagArgs &a, hipStream_t s);
 
 
-// MIXED form (mixed.hip).
+// MIXED form (mixed.hip, mixed_body.h): FP32 eigenbasis solves refined with
+// FP64 residuals, both on the matrix cores.
 struct MixedSetupArgs {
-    int n, NP, ldx, modes;
+    int n, NP, modes;
     const double *shift, *mu, *w0, *wN;
-    const double *VT, *Vinv;    // [NP][NP]: VT[k][i] = V[i][k], Vinv row-major
-    const int *slot;            // [modes]: >= 0 FP32 slot, < 0 FP64 slot -(s+1)
-    float *X32;                 // [n32][n][ldx]  X^T, i fastest
-    double *X64;                // [n64][n][ldx]
-    double *UL, *UR;            // [modes][n]
+    const double *VT;           // [NP][NP]: VT[k][i] = V[i][k]
+    double *UL, *UR;            // [modes][n] homogeneous solutions (interior)
 };
 struct MixedArgs {
-    int nx, M, K, N, n, ne, ldx, sweeps, modes;
+    int nx, M, K, N, n, ne, sweeps;
     long long xs;               // spectrum x-row stride (>= M * K)
     double *spec;               // real plane; imaginary plane at spec + plane_off
     long long plane_off;
     const double *bc_re, *bc_im, *shift;
-    const double *DzzT;         // [n][n], DzzT[j*n + i] = Dzz[i+1][j+1]
-    const double *Dz;           // [N][N]
-    const float *X32;
-    const double *X64;
-    const int *slot;
+    const double *mu;           // [n] eigenvalues of Dzz_int
+    const float *VinvT32, *VT32;  // [NP][NP]: Vinv^T, V^T in FP32 (zero padded)
+    const double *DzzTP;        // [NP][NP]: DzzTP[k][j] = Dzz[j+1][k+1] (zero padded)
+    const double *Dzz, *Dz;     // [N][N] scaled element matrices
     const double *UL, *UR, *hom;
     double alpha_l, beta_l, alpha_r, beta_r;
-    double *hist;               // [(sweeps+1)][modes]
+    double *hist;               // [sweeps][hist_rows]: |r| / |b| per row before each sweep, or null
+    long long hist_rows;        // nx * M * 2 * ne
     int periodic;               // z periodic: cyclic interface system, no face data
     double *closure;            // periodic z: [re|im][nx * M] closure coefficients, or null
 };
 int launch_mixed_setup(const MixedSetupArgs &a, hipStream_t s);
 int launch_mixed_modes(const MixedArgs &a, hipStream_t s);
+// Tile rows of the MIXED mode kernel for padded size NP.
+int mixed_tile_rows(int NP);
 
 struct LuFactorArgs {
     double *A;
@@ -225,11 +225,9 @@ struct hz_plan {
     double *d_xspec = nullptr;                        // half spectrum of the x/y derivatives
     size_t xspec_n = 0;
     // MIXED form
-    int ldx = 0, n32 = 0, n64 = 0, sweeps = 0;
-    double condV = 0.0;
-    float *d_X32 = nullptr;
-    double *d_X64 = nullptr, *d_UL = nullptr, *d_UR = nullptr, *d_hist = nullptr;
-    int *d_slot = nullptr;
+    int sweeps = 0;
+    float *d_VinvT32 = nullptr, *d_VT32 = nullptr;
+    double *d_DzzTP = nullptr, *d_UL = nullptr, *d_UR = nullptr, *d_hist = nullptr;
     // stage profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;      // events in use, per solve: nst+1

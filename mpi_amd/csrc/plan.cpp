@@ -77,10 +77,10 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
     for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
-                      p->d_Vinvrow, p->d_partial, p->d_X64, p->d_UL, p->d_UR, p->d_hist, p->d_closure})
+                      p->d_Vinvrow, p->d_partial, p->d_DzzTP, p->d_UL, p->d_UR, p->d_hist, p->d_closure})
         (void)hipFree(q);
-    (void)hipFree(p->d_X32);
-    (void)hipFree(p->d_slot);
+    (void)hipFree(p->d_VinvT32);
+    (void)hipFree(p->d_VT32);
     (void)hipFree(p->d_xspec);
     (void)hipFree(p->d_status);
     if (p->h_lambda_bar) (void)hipHostFree(p->h_lambda_bar);
@@ -114,68 +114,19 @@ int compute_homogeneous(hz_plan *p)
     return HZ_OK;
 }
 
-// MIXED form: which modes keep FP64 operators, and the operators themselves.
-// cond(Dzz_int - sigma I) <= cond(V) max|mu - sigma| / min|mu - sigma|; FP32
-// refinement contracts the error by at most about cond * 2^-24 per sweep
-// (the measured contraction is far smaller: 3e-6 for modes where this bound
-// is 0.1), so modes where the bound exceeds kMixedRate -- FP32 refinement
-// may then stall -- keep FP64 (or, if the caller gave one, modes whose shift
-// is below desc.mixed_fp64_cond).
-constexpr double kMixedRate = 1.0;
-
+// MIXED form: the homogeneous solutions ul, ur of every mode in FP64 (once
+// per lambda; the assembly of several elements or Robin faces needs them).
 int mixed_operators(hz_plan *p)
 {
-    const int n = p->n;
-    std::vector<int> slot(p->modes);
-    int n32 = 0, n64 = 0;
-    for (int m = 0; m < p->modes; ++m) {
-        const double sig = p->shift[m];
-        bool fp64;
-        if (p->d.mixed_fp64_cond > 0.0) {
-            fp64 = sig < p->d.mixed_fp64_cond;
-        } else {
-            double lo = INFINITY, hi = 0.0;
-            for (int k = 0; k < n; ++k) {
-                const double d = std::fabs(p->mu[k] - sig);
-                lo = std::min(lo, d);
-                hi = std::max(hi, d);
-            }
-            fp64 = !(p->condV * hi / lo * 0x1p-24 <= kMixedRate);
-        }
-        slot[m] = fp64 ? -(++n64) : n32++;
-    }
-    const size_t per = (size_t)n * p->ldx;
-    if (n32 != p->n32) {
-        (void)hipFree(p->d_X32);
-        p->d_X32 = nullptr;
-        p->n32 = 0;
-        int st = dmalloc(&p->d_X32, per * n32);
-        if (st) return st;
-        p->n32 = n32;
-    }
-    if (n64 != p->n64) {
-        (void)hipFree(p->d_X64);
-        p->d_X64 = nullptr;
-        p->n64 = 0;
-        int st = dmalloc(&p->d_X64, per * n64);
-        if (st) return st;
-        p->n64 = n64;
-    }
-    HZ_HIP(hipMemcpy(p->d_slot, slot.data(), sizeof(int) * p->modes, hipMemcpyHostToDevice));
     MixedSetupArgs a{};
-    a.n = n;
+    a.n = p->n;
     a.NP = p->NP;
-    a.ldx = p->ldx;
     a.modes = p->modes;
     a.shift = p->d_shift;
     a.mu = p->d_mu;
     a.w0 = p->d_w0;
     a.wN = p->d_wN;
     a.VT = p->d_VT;
-    a.Vinv = p->d_Vinvrow;
-    a.slot = p->d_slot;
-    a.X32 = p->d_X32;
-    a.X64 = p->d_X64;
     a.UL = p->d_UL;
     a.UR = p->d_UR;
     int st = launch_mixed_setup(a, nullptr);
@@ -184,29 +135,30 @@ int mixed_operators(hz_plan *p)
     return HZ_OK;
 }
 
+// MIXED form: FP32 copies of the eigenbasis (V^{-T}, V^T) and the padded FP64
+// Dzz_int^T of the residual, all NP x NP; the history and homogeneous
+// solution buffers.
 int setup_mixed(hz_plan *p)
 {
-    p->ldx = (p->n + 3) / 4 * 4;
-    p->sweeps = p->d.refine_sweeps > 0 ? p->d.refine_sweeps : (p->d.refine_sweeps < 0 ? 0 : 3);
+    p->sweeps = p->d.refine_sweeps > 0 ? p->d.refine_sweeps : (p->d.refine_sweeps < 0 ? 0 : 2);
     if (p->sweeps > 16) return set_error(HZ_ERR_ARG, "refine_sweeps must be <= 16");
-    if (p->ne > 4) return set_error(HZ_ERR_UNSUPPORTED, "MIXED form: at most 4 elements");
-    // 1-norm condition number of V (plan time, host)
-    const int n = p->n;
-    double nv = 0.0, ni = 0.0;
-    for (int j = 0; j < n; ++j) {
-        double cv = 0.0, ci = 0.0;
-        for (int i = 0; i < n; ++i) {
-            cv += std::fabs(p->V[(size_t)i * n + j]);
-            ci += std::fabs(p->Vinv[(size_t)i * n + j]);
+    if (2 * p->ne > mixed_tile_rows(p->NP))
+        return set_error(HZ_ERR_UNSUPPORTED, "MIXED form: too many elements for one tile (at most 8, or 16 "
+                                             "when N-2 rounds up to 32 or 96)");
+    const int n = p->n, N = p->N, NP = p->NP;
+    std::vector<float> VinvT32((size_t)NP * NP, 0.0f), VT32((size_t)NP * NP, 0.0f);
+    std::vector<double> DzzTP((size_t)NP * NP, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            VinvT32[(size_t)j * NP + i] = (float)p->Vinv[(size_t)i * n + j];
+            VT32[(size_t)j * NP + i] = (float)p->V[(size_t)i * n + j];
+            DzzTP[(size_t)j * NP + i] = p->Dzz[(size_t)(i + 1) * N + (j + 1)];
         }
-        nv = std::max(nv, cv);
-        ni = std::max(ni, ci);
-    }
-    p->condV = nv * ni;
     int st;
-    if ((st = dmalloc(&p->d_slot, p->modes)) || (st = dmalloc(&p->d_UL, (size_t)p->modes * n)) ||
+    if ((st = upload(&p->d_VinvT32, VinvT32)) || (st = upload(&p->d_VT32, VT32)) ||
+        (st = upload(&p->d_DzzTP, DzzTP)) || (st = dmalloc(&p->d_UL, (size_t)p->modes * n)) ||
         (st = dmalloc(&p->d_UR, (size_t)p->modes * n)) ||
-        (st = dmalloc(&p->d_hist, (size_t)(p->sweeps + 1) * p->modes)))
+        (st = dmalloc(&p->d_hist, (size_t)std::max(p->sweeps, 1) * p->nx * p->M * 2 * p->ne)))
         return st;
     return HZ_OK;
 }
@@ -220,7 +172,7 @@ int compute_shifts(hz_plan *p)
     HZ_HIP(hipMemcpy(p->d_shift, p->shift.data(), sizeof(double) * p->modes, hipMemcpyHostToDevice));
     int st = compute_homogeneous(p);
     if (st) return st;
-    if (p->d_slot) return mixed_operators(p);
+    if (p->d.form == HZ_FORM_MIXED && p->d_UL) return mixed_operators(p);
     return HZ_OK;
 }
 
@@ -766,20 +718,19 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.N = p->N;
         a.n = p->n;
         a.ne = p->ne;
-        a.ldx = p->ldx;
         a.sweeps = p->sweeps;
-        a.modes = p->modes;
         a.spec = p->spec.re;
         a.plane_off = p->spec.im - p->spec.re;
         a.xs = p->xs;
         a.bc_re = p->bc.re;
         a.bc_im = p->bc.im;
         a.shift = p->d_shift;
-        a.DzzT = p->d_DzzT;
+        a.mu = p->d_mu;
+        a.VinvT32 = p->d_VinvT32;
+        a.VT32 = p->d_VT32;
+        a.DzzTP = p->d_DzzTP;
+        a.Dzz = p->d_Dzz;
         a.Dz = p->d_Dz;
-        a.X32 = p->d_X32;
-        a.X64 = p->d_X64;
-        a.slot = p->d_slot;
         a.UL = p->d_UL;
         a.UR = p->d_UR;
         a.hom = p->d_hom;
@@ -787,7 +738,8 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         a.beta_l = p->d.beta_l;
         a.alpha_r = p->d.alpha_r;
         a.beta_r = p->d.beta_r;
-        a.hist = p->d_hist;
+        a.hist = p->sweeps > 0 ? p->d_hist : nullptr;
+        a.hist_rows = (long long)p->nx * p->M * 2 * p->ne;
         a.periodic = p->d.periodic_z;
         a.closure = p->d_closure;
         st = launch_mixed_modes(a, s);
@@ -948,13 +900,14 @@ int hz_mixed_history(const hz_plan *p, double *rel_residual, int max_sweeps, int
 {
     if (!p || !rel_residual || !sweeps) return set_error(HZ_ERR_ARG, "null argument");
     if (p->d.form != HZ_FORM_MIXED) return set_error(HZ_ERR_UNSUPPORTED, "not a MIXED plan");
-    const int S = p->sweeps + 1;
-    std::vector<double> h((size_t)S * p->modes);
+    const int S = p->sweeps;
+    const size_t rows = (size_t)p->nx * p->M * 2 * p->ne;
+    std::vector<double> h((size_t)S * rows);
     HZ_HIP(hipDeviceSynchronize());
-    HZ_HIP(hipMemcpy(h.data(), p->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    if (S) HZ_HIP(hipMemcpy(h.data(), p->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < S && s < max_sweeps; ++s) {
         double v = 0.0;
-        for (int m = 0; m < p->modes; ++m) v = std::max(v, h[(size_t)s * p->modes + m]);
+        for (size_t r = 0; r < rows; ++r) v = std::max(v, h[(size_t)s * rows + r]);
         rel_residual[s] = v;
     }
     *sweeps = std::min(S, max_sweeps);
@@ -1011,7 +964,7 @@ int hz_plan_periodic_closure(const hz_plan *p, double *alpha)
 int hz_mixed_fp64_modes(const hz_plan *p, int *fp64_modes)
 {
     if (!p || !fp64_modes) return set_error(HZ_ERR_ARG, "null argument");
-    *fp64_modes = p->n64;
+    *fp64_modes = 0;  // every mode runs the FP32 eigenbasis solve (its refinement converges for all shifts)
     return HZ_OK;
 }
 

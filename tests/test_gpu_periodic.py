@@ -27,7 +27,7 @@ def plan_for(d, ne, form):
 
 CASES = [("shipped_np1", "plain"), ("shipped_np1", "diag")] + [
     (name, form) for name in ("random_64_N33", "random_128_N129", "random_32_N9")
-    for form in ("plain", "diag", "mixed")]  # MIXED takes at most 4 elements: not the 25-element shipped case
+    for form in ("plain", "diag", "mixed")]  # MIXED takes at most 8 elements: not the 25-element shipped case
 
 
 @pytest.mark.parametrize("name,form", CASES)
@@ -73,8 +73,8 @@ def test_periodic_rejects_unsupported(cuda):
     import torch
     from mpi_amd._lib import HzError
     from mpi_amd.solver import HelmholtzPlan
-    with pytest.raises(HzError):  # MIXED: at most 4 elements
-        HelmholtzPlan(8, 8, 9, 5, form="mixed", periodic_z=True)
+    with pytest.raises(HzError):  # MIXED: a column's 2 x 17 element rows exceed its 32-row tile
+        HelmholtzPlan(8, 8, 9, 17, form="mixed", periodic_z=True)
     p = HelmholtzPlan(8, 8, 9, 2, form="diag", periodic_z=True)
     fbar, glbar, grbar, _ = p.adjoint(torch.zeros((8, 8, p.nz), dtype=torch.float64, device="cuda"))
     assert glbar is None and grbar is None and float(fbar.abs().max()) == 0.0
