@@ -281,18 +281,19 @@ def batch_roofline(plans, batch_ms, ceilings):
             "frac_of_attainable": (t_mode + t_fft) / batch_ms}
 
 
-def time_batch(steps=20, ceilings=None):
+def time_batch(steps=20, ceilings=None, form="diag"):
     """Many-problem workload: 512 independent problems, nx, ny uniform in
     {32, 64, 128}, N odd uniform in [17, 129], lambda log-uniform in
     [1e-2, 1e3], 16 random modes each; one hz_batch (autotuned, graph
-    replay) against the same problems solved one plan at a time."""
+    replay) against the same problems solved one plan at a time.  form:
+    "diag", or "mixed" (FP32 eigenbasis solves with FP64 refinement)."""
     import torch
     from mpi_amd import workloads as wl
     from mpi_amd.solver import BatchSolver, HelmholtzPlan
     specs = wl.batch_specs(512, seed=3)
     plans, fs, gls, grs, us = [], [], [], [], []
     for sp in specs:
-        p = HelmholtzPlan(sp["nx"], sp["ny"], sp["N"], 1, lam=sp["lam"], form="diag")
+        p = HelmholtzPlan(sp["nx"], sp["ny"], sp["N"], 1, lam=sp["lam"], form=form)
         f, gl, gr = wl.batch_fields(sp, p.z_nodes() - 1.0)
         plans.append(p)
         fs.append(torch.from_numpy(f).cuda())
@@ -320,11 +321,12 @@ def time_batch(steps=20, ceilings=None):
     t3 = time.perf_counter()
     batch_ms = 1e3 * (t1 - t0) / steps
     seq_ms = 1e3 * (t3 - t2) / steps
-    out = {"workload": "512 problems, nx,ny in {32,64,128}, N odd in [17,129], lambda in [1e-2,1e3], DIAG",
+    out = {"workload": "512 problems, nx,ny in {32,64,128}, N odd in [17,129], lambda in [1e-2,1e3], "
+                       + form.upper(),
            "points": int(sum(p.nx * p.ny * p.nz for p in plans)), "batch_ms": batch_ms,
            "problems_per_s": 512 / (batch_ms * 1e-3), "one_plan_at_a_time_ms": seq_ms,
            "speedup": seq_ms / batch_ms, "tuning": b.tuning()}
-    if ceilings:
+    if ceilings and form == "diag":
         out["roofline"] = batch_roofline(plans, batch_ms, ceilings)
     b.close()
     for p in plans:
@@ -553,6 +555,7 @@ def main():
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
         line["allen_cahn"] = time_allen_cahn()
         line["batch"] = time_batch(ceilings=roof["attainable"])
+        line["batch_mixed"] = time_batch(ceilings=roof["attainable"], form="mixed")
         line["precision"] = time_precision()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(f, gl, gr)

@@ -207,7 +207,7 @@ int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int 
 
 /* Many independent problems of uneven size solved together (no
  * counterpart in the reference, which solves one problem per process).
- * plans[p] are DIAG-form plans (distinct; nx, ny 1 or in [16, 256]; N - 2 <= 128;
+ * plans[p] are DIAG- or MIXED-form plans, mixed freely (distinct; nx, ny 1 or in [16, 256]; N - 2 <= 128;
  * ne <= 16; Robin or periodic z); f[p], g_l[p], g_r[p], u[p] are device
  * arrays in each plan's layout (g_l[p], g_r[p] NULL for a periodic plan).  Each stage (transform passes, mode solve) is one launch per kind
  * of task over all problems.  The first solve of a batch shape autotunes the

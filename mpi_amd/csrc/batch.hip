@@ -1,8 +1,8 @@
-// batch.hip -- many independent DIAG-form problems of uneven size (nx, ny,
+// batch.hip -- many independent DIAG- or MIXED-form problems of uneven size (nx, ny,
 // N, ne, lambda, faces) solved together.
 //
 // Every problem is the five-stage pipeline of a solve:
-//   0 R2C along y (+ faces)   1 C2C along x (+ faces)   2 diagonalised mode solve
+//   0 R2C along y (+ faces)   1 C2C along x (+ faces)   2 mode solve (diagonalised, or MIXED)
 //   3 C2C^-1 along x          4 C2R along y
 // split into tasks: a tile of spectral columns of the mode solve, or for a
 // transform pass one uniform-base tile of the standalone transform kernels
@@ -24,8 +24,8 @@
 // The autotuner times the enabled variants (mode-solve tile height 32 or 64
 // rows) on the first solve of a batch shape, keeps the fastest, and captures
 // it in a HIP graph that later solves replay.
-#include "diag_body.h"
 #include "fft_body.h"
+#include "mixed_body.h"
 
 #include <array>
 #include <map>
@@ -39,10 +39,11 @@ constexpr int kStages = 5;
 
 struct BatchProblem {
     FftArgs f[4];    // passes 0..3 of xy_pass_args
-    DiagArgs d;
+    DiagArgs d;      // DIAG plans
+    MixedArgs m;     // MIXED plans
     int ly, lx;      // transform lengths along y, x
     int ntask[kStages];
-    int dkind;       // 2 * (NP / 32 - 1) + general, NP = 32, 64, 96, 128
+    int dkind;       // 2 * (NP / 32 - 1) + general (NP = 32, 64, 96, 128), + 8 for a MIXED plan
 };
 
 namespace {
@@ -219,6 +220,15 @@ __global__ __launch_bounds__(256) void batch_diag_list(const BatchProblem *__res
     diag_tile<NJ, RT, GENERAL, false>(P[t.x].d, t.y, sh);
 }
 
+template <int NJ, int RT, bool GENERAL>
+__global__ __launch_bounds__(256, 2) void batch_mixed_list(const BatchProblem *__restrict__ P,
+                                                           const int2 *__restrict__ list)
+{
+    __shared__ MixedShared<NJ, RT, GENERAL> sh;
+    const int2 t = list[blockIdx.x];
+    mixed_tile<NJ, RT, GENERAL>(P[t.x].m, t.y, sh);
+}
+
 template <int RT>
 constexpr size_t batch_lds()
 {
@@ -291,7 +301,9 @@ std::vector<int> task_table(const hz_batch *b, int rt, std::vector<std::array<in
     counts->assign(n, {});
     for (int p = 0; p < n; ++p) {
         const hz_plan *pl = b->plans[p];
-        const int fc_per_tile = rt / (2 * pl->ne);
+        // MIXED tiles have a fixed height per padded size
+        const int rows = pl->d.form == HZ_FORM_MIXED ? mixed_tile_rows(diag_padded(pl->n)) : rt;
+        const int fc_per_tile = rows / (2 * pl->ne);
         const int nfc = pl->nx * pl->M;
         (*counts)[p] = {pl->nx, pl->M, (nfc + fc_per_tile - 1) / fc_per_tile, pl->M, pl->nx};
     }
@@ -330,12 +342,15 @@ int fill_problems(hz_batch *b, const double *const *f, const double *const *gl, 
                               nullptr, xs);
         q.f[3] = xy_pass_args(pl, 3, nullptr, pl->spec.re, pl->spec.im, u[p], pl->K, nullptr, nullptr, nullptr,
                               nullptr, xs);
-        q.d = diag_args(pl);
+        if (pl->d.form == HZ_FORM_MIXED)
+            q.m = mixed_args(pl);
+        else
+            q.d = diag_args(pl);
         q.ly = pl->ny;
         q.lx = pl->nx;
         for (int s = 0; s < kStages; ++s) q.ntask[s] = counts[p][s];
         const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
-        q.dkind = 2 * (diag_padded(pl->n) / 32 - 1) + (general ? 1 : 0);
+        q.dkind = 2 * (diag_padded(pl->n) / 32 - 1) + (general ? 1 : 0) + (pl->d.form == HZ_FORM_MIXED ? 8 : 0);
     }
     return HZ_OK;
 }
@@ -359,10 +374,29 @@ void launch_fft_list(hipStream_t s, const BatchProblem *P, const int2 *list, int
     }
 }
 
+void launch_mixed_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int kind)
+{
+    const dim3 g(count), b(256);
+    switch (kind) {  // MIXED tile heights: mixed_tile_rows
+    case 0: hipLaunchKernelGGL((batch_mixed_list<2, 32, false>), g, b, 0, s, P, list); break;
+    case 1: hipLaunchKernelGGL((batch_mixed_list<2, 32, true>), g, b, 0, s, P, list); break;
+    case 2: hipLaunchKernelGGL((batch_mixed_list<4, 16, false>), g, b, 0, s, P, list); break;
+    case 3: hipLaunchKernelGGL((batch_mixed_list<4, 16, true>), g, b, 0, s, P, list); break;
+    case 4: hipLaunchKernelGGL((batch_mixed_list<6, 32, false>), g, b, 0, s, P, list); break;
+    case 5: hipLaunchKernelGGL((batch_mixed_list<6, 32, true>), g, b, 0, s, P, list); break;
+    case 6: hipLaunchKernelGGL((batch_mixed_list<8, 16, false>), g, b, 0, s, P, list); break;
+    default: hipLaunchKernelGGL((batch_mixed_list<8, 16, true>), g, b, 0, s, P, list); break;
+    }
+}
+
 template <int RT>
 void launch_diag_list(hipStream_t s, const BatchProblem *P, const int2 *list, int count, int kind)
 {
     const dim3 g(count), b(256);
+    if (kind >= 8) {
+        launch_mixed_list(s, P, list, count, kind - 8);
+        return;
+    }
     switch (kind) {
     case 0: hipLaunchKernelGGL((batch_diag_list<2, RT, false>), g, b, 0, s, P, list); break;
     case 1: hipLaunchKernelGGL((batch_diag_list<2, RT, true>), g, b, 0, s, P, list); break;
@@ -487,7 +521,8 @@ std::string batch_shape(const std::vector<hz_plan *> &plans)
     for (const hz_plan *pl : plans) {
         const bool general = pl->ne > 1 || pl->d.alpha_l != 0.0 || pl->d.alpha_r != 0.0 || pl->d.periodic_z;
         shape += std::to_string(pl->nx) + "x" + std::to_string(pl->ny) + "x" + std::to_string(pl->N) + "e" +
-                 std::to_string(pl->ne) + (pl->d.periodic_z ? "p;" : general ? "g;" : ";");
+                 std::to_string(pl->ne) + (pl->d.form == HZ_FORM_MIXED ? "m" + std::to_string(pl->sweeps) : "") +
+                 (pl->d.periodic_z ? "p;" : general ? "g;" : ";");
     }
     return shape;
 }
@@ -548,7 +583,8 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
         if (!pl) return set_error(HZ_ERR_ARG, "null plan");
         for (int q = 0; q < p; ++q)
             if (plans[q] == pl) return set_error(HZ_ERR_ARG, "a plan appears twice in the batch");
-        if (pl->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG-form plans only");
+        if (pl->d.form != HZ_FORM_DIAG && pl->d.form != HZ_FORM_MIXED)
+            return set_error(HZ_ERR_UNSUPPORTED, "batch: DIAG- or MIXED-form plans only");
         if (pl->device != dev) return set_error(HZ_ERR_ARG, "batch: plans on another device");
         if (pl->n > 128) return set_error(HZ_ERR_UNSUPPORTED, "batch: N-2 must be <= 128");
         if ((pl->nx != 1 && (pl->nx < 16 || pl->nx > 256)) || (pl->ny != 1 && (pl->ny < 16 || pl->ny > 256)))

@@ -190,6 +190,8 @@ FftArgs xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, d
 inline long long spectrum_xs(int M, int K) { return ((long long)M * K + 15) / 16 * 16; }
 // Arguments of the forward diagonalised mode solve on `p`'s workspace.
 DiagArgs diag_args(const hz_plan *p);
+// Arguments of the MIXED mode solve on `p`'s workspace.
+MixedArgs mixed_args(const hz_plan *p);
 }  // namespace hz
 
 // ---- the plan ----------------------------------------------------------------

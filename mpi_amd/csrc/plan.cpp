@@ -241,6 +241,43 @@ int setup_diag(hz_plan *p)
 
 }  // namespace
 
+// Arguments of the MIXED mode solve on `p`'s workspace.
+MixedArgs hz::mixed_args(const hz_plan *p)
+{
+    MixedArgs a{};
+    a.nx = p->nx;
+    a.M = p->M;
+    a.K = p->K;
+    a.N = p->N;
+    a.n = p->n;
+    a.ne = p->ne;
+    a.sweeps = p->sweeps;
+    a.spec = p->spec.re;
+    a.plane_off = p->spec.im - p->spec.re;
+    a.xs = p->xs;
+    a.bc_re = p->bc.re;
+    a.bc_im = p->bc.im;
+    a.shift = p->d_shift;
+    a.mu = p->d_mu;
+    a.VinvT32 = p->d_VinvT32;
+    a.VT32 = p->d_VT32;
+    a.DzzTP = p->d_DzzTP;
+    a.Dzz = p->d_Dzz;
+    a.Dz = p->d_Dz;
+    a.UL = p->d_UL;
+    a.UR = p->d_UR;
+    a.hom = p->d_hom;
+    a.alpha_l = p->d.alpha_l;
+    a.beta_l = p->d.beta_l;
+    a.alpha_r = p->d.alpha_r;
+    a.beta_r = p->d.beta_r;
+    a.hist = p->sweeps > 0 ? p->d_hist : nullptr;
+    a.hist_rows = (long long)p->nx * p->M * 2 * p->ne;
+    a.periodic = p->d.periodic_z;
+    a.closure = p->d_closure;
+    return a;
+}
+
 // Arguments of the four line-transform passes of a solve (see hz_internal.h).
 FftArgs hz::xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, double *im, double *out,
                          int lanes, const double *gl, const double *gr, double *gl_out, double *gr_out,
@@ -711,37 +748,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         st = launch_diag_modes(a, s);
         rec.mark("diag_modes");
     } else if (p->d.form == HZ_FORM_MIXED) {
-        MixedArgs a{};
-        a.nx = p->nx;
-        a.M = p->M;
-        a.K = p->K;
-        a.N = p->N;
-        a.n = p->n;
-        a.ne = p->ne;
-        a.sweeps = p->sweeps;
-        a.spec = p->spec.re;
-        a.plane_off = p->spec.im - p->spec.re;
-        a.xs = p->xs;
-        a.bc_re = p->bc.re;
-        a.bc_im = p->bc.im;
-        a.shift = p->d_shift;
-        a.mu = p->d_mu;
-        a.VinvT32 = p->d_VinvT32;
-        a.VT32 = p->d_VT32;
-        a.DzzTP = p->d_DzzTP;
-        a.Dzz = p->d_Dzz;
-        a.Dz = p->d_Dz;
-        a.UL = p->d_UL;
-        a.UR = p->d_UR;
-        a.hom = p->d_hom;
-        a.alpha_l = p->d.alpha_l;
-        a.beta_l = p->d.beta_l;
-        a.alpha_r = p->d.alpha_r;
-        a.beta_r = p->d.beta_r;
-        a.hist = p->sweeps > 0 ? p->d_hist : nullptr;
-        a.hist_rows = (long long)p->nx * p->M * 2 * p->ne;
-        a.periodic = p->d.periodic_z;
-        a.closure = p->d_closure;
+        MixedArgs a = mixed_args(p);
         st = launch_mixed_modes(a, s);
         rec.mark("mixed_modes");
     } else {

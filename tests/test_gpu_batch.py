@@ -34,13 +34,13 @@ MIXED_SET = [(16, 32, 17, 1, 0.5, False), (32, 16, 33, 2, 3.0, True), (64, 64, 6
              (16, 256, 9, 4, 7.0, True)]
 
 
-def make_set(spec):
+def make_set(spec, forms=None):
     from mpi_amd.solver import HelmholtzPlan
     plans, inputs, descs = [], [], []
     for k, (nx, ny, N, ne, lam, robin) in enumerate(spec):
         al, bl, ar, br = (0.7, 1.3, 0.4, 2.0) if robin else (0.0, 1.0, 0.0, 1.0)
         p = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam,
-                          alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form="diag")
+                          alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form=forms[k] if forms else "diag")
         rng = np.random.default_rng(100 + k)
         f = rng.standard_normal((nx, ny, p.nz))
         gl = rng.standard_normal((nx, ny))
@@ -287,4 +287,32 @@ def test_batch_follows_plan_changes(cuda):
     assert rel(after[1], before[1]) <= 1e-13  # the untouched plan
     plans[0].set_robin(0.0, 1.0, 0.0, 1.0)  # back to Dirichlet
     check_against_single("back to Dirichlet")
+    b.close()
+
+
+def test_batch_with_mixed_plans(cuda):
+    """MIXED plans (FP32 eigenbasis solves refined with FP64 residuals) batch
+    with DIAG plans: each problem equals its single-plan solve (the same tile
+    body) and the oracle to the rounding-spread bar."""
+    from mpi_amd.solver import BatchSolver
+    forms = ["mixed", "diag", "mixed", "mixed", "mixed", "diag", "mixed"]
+    plans, inputs, descs = make_set(MIXED_SET, forms)
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [dev(g) for _, g, _ in inputs]
+    grs = [dev(g) for _, _, g in inputs]
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    b = BatchSolver(plans)
+    for rep in range(2):  # autotune + capture, then a replay
+        us = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(us, want)):
+            e = rel(u.cpu().numpy(), w)
+            assert e <= 1e-13, f"rep {rep} problem {k} ({forms[k]}): {e:.3e}"
+    for k in (0, 2, 3, 6):
+        f, gl, gr = inputs[k]
+        uo = po.solve(descs[k], f, gl, gr)
+        e = rel(us[k].cpu().numpy(), uo)
+        bar = max(1e-12, 2 * rel(po.solve(descs[k], f, gl, gr, fma=True), uo))
+        print(f"mixed problem {k}: rel err {e:.2e}, bar {bar:.2e}")
+        assert e <= bar
     b.close()
