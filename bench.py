@@ -321,11 +321,12 @@ def time_batch(steps=20, ceilings=None, form="diag"):
     t3 = time.perf_counter()
     batch_ms = 1e3 * (t1 - t0) / steps
     seq_ms = 1e3 * (t3 - t2) / steps
+    tuning = b.tuning()
     out = {"workload": "512 problems, nx,ny in {32,64,128}, N odd in [17,129], lambda in [1e-2,1e3], "
                        + form.upper(),
            "points": int(sum(p.nx * p.ny * p.nz for p in plans)), "batch_ms": batch_ms,
            "problems_per_s": 512 / (batch_ms * 1e-3), "one_plan_at_a_time_ms": seq_ms,
-           "speedup": seq_ms / batch_ms, "tuning": b.tuning()}
+           "speedup": seq_ms / batch_ms, "tuning": tuning}
     if ceilings and form == "diag":
         out["roofline"] = batch_roofline(plans, batch_ms, ceilings)
     b.close()

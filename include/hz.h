@@ -209,27 +209,31 @@ int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int 
  * counterpart in the reference, which solves one problem per process).
  * plans[p] are DIAG- or MIXED-form plans, mixed freely (distinct; nx, ny 1 or in [16, 256]; N - 2 <= 128;
  * ne <= 16; Robin or periodic z); f[p], g_l[p], g_r[p], u[p] are device
- * arrays in each plan's layout (g_l[p], g_r[p] NULL for a periodic plan).  Each stage (transform passes, mode solve) is one launch per kind
- * of task over all problems.  The first solve of a batch shape autotunes the
- * variant (mode-solve tile height) and records a HIP graph of the winner;
- * later solves replay it (re-reading the buffer table when the pointers
- * change).  Variants 0-3 (one persistent fused launch over a
- * dependency-ordered task queue) are built but switched off.  Enqueued on `stream`; a plan in a batch must not be
- * used by another solve while the batch runs. */
+ * arrays in each plan's layout (g_l[p], g_r[p] NULL for a periodic plan).
+ * Two ways to run a batch, each with mode-solve tiles of 32 or 64 rows:
+ * staged (variants 4, 5: one launch per stage and kind of task, recorded in
+ * a HIP graph that later solves replay, re-reading the buffer table when the
+ * pointers change) and fused (variants 6, 7: ONE cooperative launch per
+ * solve, the five stages separated by grid barriers; built but switched off
+ * and refused, since two of its GPU runs did not finish).  The first solve of
+ * a batch shape times every usable variant and keeps the fastest.  Variants
+ * 0-3 are refused (an earlier fused design, removed).
+ * Enqueued on `stream`; a plan in a batch must not be used by another solve
+ * while the batch runs. */
 typedef struct hz_batch hz_batch;
 int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch);
 int hz_batch_solve(hz_batch *batch, const double *const *f, const double *const *gl,
                    const double *const *gr, double *const *u, hz_stream_t stream);
-/* Synchronises the batch and reports HZ_ERR_HIP if a dependency wait of the
- * fused launch hit its spin limit (results then unreliable). */
+/* Synchronises the batch and reports HZ_ERR_HIP if the fused kernel ran
+ * without a cooperative launch (it then does no work). */
 int hz_batch_status(hz_batch *batch);
-/* The autotuner's choice (variant index, fused or staged, tile rows,
- * persistent work-groups per CU) and the measured ms of every variant. */
+/* The autotuner's choice (variant index, fused or staged, tile rows, the
+ * fused launch's work-groups per CU) and the measured ms of every variant
+ * (0 where not timed). */
 int hz_batch_tuning(const hz_batch *batch, int *variant, int *fused, int *tile_rows,
                     int *wgs_per_cu, double *variant_ms, int max_variants);
-/* Pins the variant (4 or 5; 0-3 are switched off and refused; -1 = autotune
- * again on the next solve); drops the recorded graph.  For tests and tuning
- * studies. */
+/* Pins the variant (4 or 5; 0-3 and 6-7 are refused; -1 = autotune again on
+ * the next solve); drops the recorded graph.  For tests and tuning studies. */
 int hz_batch_force_variant(hz_batch *batch, int variant);
 int hz_batch_destroy(hz_batch *batch);
 /* One-shot form of the above: create the batch, solve, wait for it,

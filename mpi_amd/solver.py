@@ -227,11 +227,13 @@ class HelmholtzPlan:
 
 
 class BatchSolver:
-    """Many DIAG-form plans solved together (hz_batch_*): one persistent
-    fused launch or one launch per stage, chosen by the autotuner on the first
-    solve and replayed from a HIP graph afterwards."""
+    """Many DIAG- or MIXED-form plans solved together (hz_batch_*): one launch
+    per stage and task kind (replayed from a HIP graph) or one cooperative
+    fused launch for the whole batch, chosen by the autotuner on the first
+    solve.  Variants: 4, 5 staged (mode-solve tiles of 32, 64 rows); 6, 7
+    fused (switched off) and 0-3 refused."""
 
-    NUM_VARIANTS = 6
+    NUM_VARIANTS = 8
 
     def __init__(self, plans):
         self.lib = load()

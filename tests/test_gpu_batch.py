@@ -1,8 +1,7 @@
 """Many problems of uneven size in one batch (hz_batch_*).
 
-Every enabled variant (per-stage launches, both tile heights; the fused
-persistent variants are switched off) must reproduce each problem's
-single-plan DIAG solve, which is
+Every variant (per-stage launches or one cooperative fused launch, both
+tile heights) must reproduce each problem's single-plan DIAG solve, which is
 itself pinned to the oracle; a few problems are also checked against the
 oracle directly.  Bar: relative max-norm difference <= 1e-13 against the
 single-plan solve; against the oracle <= max(1e-12, 2 x the reference
@@ -52,12 +51,20 @@ def make_set(spec, forms=None):
     return plans, inputs, descs
 
 
-FUSED_OFF = ("fused persistent variant switched off: its first GPU run did not finish (dependency waits "
-             "ran to the spin limit); cause not established")
+FUSED_OFF = ("variants 0-3, a persistent launch whose tasks waited on per-problem completion counters, were "
+             "removed: its first GPU run did not finish and the cause was not established")
+# The cooperative fused launch (variants 6 and 7) is refused: variant 6
+# reproduced the staged results bit for bit (512 problems; periodic, 2-D and
+# 1-D sets), but variant 7's first run did not finish, nor did a run of the
+# MIXED-plan batch test below while variant 6 was timed and forced on it; the
+# cause is not established.
+COOP_OFF = ("fused cooperative launch (variants 6, 7) switched off: two GPU runs did not finish; cause not "
+            "established")
+STAGED = (4, 5)
 
 
 @pytest.mark.parametrize("variant", [pytest.param(v, marks=pytest.mark.skip(reason=FUSED_OFF)) for v in range(4)]
-                         + [4, 5])
+                         + list(STAGED) + [pytest.param(v, marks=pytest.mark.skip(reason=COOP_OFF)) for v in (6, 7)])
 def test_batch_variant_matches_single_solves(cuda, variant):
     from mpi_amd.solver import BatchSolver
     plans, inputs, _ = make_set(MIXED_SET)
@@ -144,6 +151,16 @@ def test_batch_many_problem_workload(cuda):
         worst = max(worst, rel(u.cpu().numpy(), p.solve(f, gl, gr).cpu().numpy()))
     print(f"worst relative difference over 512 problems: {worst:.3e}")
     assert worst <= 1e-13
+    # both staged variants give the same bits over the 512 problems
+    import torch
+    first = [u.clone() for u in us]
+    for v in STAGED:
+        b.force_variant(v)
+        for rep in range(2):
+            got = b.solve(fs, gls, grs)
+            b.status()
+            for k, (u, w) in enumerate(zip(got, first)):
+                assert torch.equal(u, w), f"variant {v} solve {rep}: problem {k} differs"
     b.close()
 
 
@@ -157,8 +174,9 @@ def test_batch_rejects_bad_sets(cuda):
     with pytest.raises(HzError):
         BatchSolver([p, q])
     b = BatchSolver([p])
-    with pytest.raises(HzError):
-        b.force_variant(0)  # fused variants are switched off
+    for v in (0, 3, 6, 7, 8, -2):  # 0-3 removed, 6-7 switched off; 8, -2 out of range
+        with pytest.raises(HzError):
+            b.force_variant(v)
     b.close()
 
 
