@@ -11,9 +11,7 @@
 //      the candidate at position i is |a_ik| / s_i where s_i is the max-norm
 //      of the ORIGINAL row at position i (the scale is never exchanged with
 //      the rows), the strictly-greater test makes the first maximum in
-//      position order win, |scaled pivot| <= 1e-12 is singular, and an
-//      exchange rounds both rows like the reference's a+b-b swap (matrix
-//      columns as in LUDecomposition, right-hand sides as in LUSolve).
+//      position order win, and |scaled pivot| <= 1e-12 is singular.
 //      Carrying the right-hand sides through the elimination is LUSolve's
 //      forward phase (:86-95);
 //   3. back-substitutes the right-hand sides (:97-107);
@@ -23,18 +21,17 @@
 //      (:582-587) straight into the spectrum.
 //
 // Exchanges are implicit: rows never move, every physical row carries its
-// current position, and the pivot row is broadcast from wherever it lives.
-// The mode kernel's elimination uses FMA, so its results agree with the
-// reference to round-off.  The standalone factor and solve kernels (the
-// drop-in LUDecomposition / LUSolve) round every product and difference
-// separately, in the reference's order, and reproduce it bit for bit.
+// current position, and the pivot row is found wherever it lives.  The mode
+// kernel (eliminate_modes) uses FMA and exact exchanges, so its results
+// agree with the reference to round-off.  The standalone factor and solve
+// kernels (the drop-in LUDecomposition / LUSolve) round every product and
+// difference separately, in the reference's order, apply the reference's
+// a+b-b exchange arithmetic, and reproduce it bit for bit.
 //
 // Layout: 64*TC threads.  Wave c (of TC) owns matrix columns c + TC*b
 // (b < CB) and right-hand sides c + TC*t (t < NR); lane r owns physical rows
 // r + 64*a (a < RB).  A column lives in one wave, so the pivot search is a
-// wave reduction and each step needs one work-group barrier.  Run-time slot
-// positions are reached through jump tables (lu_slots.h), never by indexing
-// a register array.
+// wave reduction (DPP and readlane, no LDS).
 #include "hz_internal.h"
 #include "lu_slots.h"
 #include "sweep.h"
@@ -68,21 +65,30 @@ __device__ __forceinline__ double sub_product_rounded(double a, double l, double
     return a - l * p;
 }
 
+// Publish buffers of the mode kernel's elimination: a wave is at most four
+// positions behind the latest published one, so 8 buffers are never
+// overwritten while a wave may still read them.
+constexpr int kPub = 8;
+// Bound on a wave's wait for a published position (s_sleep trips of 64
+// cycles, ~2 ms): past it the mode reports HZ_ERR_HIP instead of hanging.
+constexpr int kPubSpin = 1 << 16;
+
 template <int RB, int CB, int NR, int TC>
 struct Smem {
     static constexpr int ROWS = WAVE * RB;
     double scale[ROWS];        // s by position, LU_Decomposition.cpp:20-29
-    double lbuf[2][ROWS];      // multipliers by physical row
+    double lbuf[kPub][ROWS];   // multipliers by physical row
     double part[TC][ROWS];
-    double prow[TC][CB];       // per-wave staging of the pivot row
-    double krow[TC][CB];       // ... and of the row it is exchanged with
-    double prowy[TC][NR];
-    double krowy[TC][NR];
-    int info[2][3];            // {npivot[k], physical pivot row, physical row at position k}
+    alignas(16) double prow[TC][CB];   // per-wave staging of the pivot row
+    alignas(16) double krow[TC][CB];   // ... and of the row it is exchanged with
+    alignas(16) double prowy[TC][NR];
+    alignas(16) double krowy[TC][NR];
+    int info[kPub][3];         // {npivot[k], physical pivot row, physical row at position k}
+    int flag[kPub];            // k + 1 once position k is published in buffer k % kPub
     int phys_of_pos[ROWS];
     double udiag[ROWS];        // U diagonal by position
     double det;
-    int singular;
+    int singular;  // 1: a pivot failed the reference's test; 2: a wait ran to its bound
 };
 
 // Elimination core.  A[a][b]: physical row lane + 64a, matrix column
@@ -323,6 +329,303 @@ __device__ __forceinline__ void eliminate(double (&A)[RB][CB], double (&Y)[RB][N
     __syncthreads();
 }
 
+// ---- the mode kernel's elimination ------------------------------------------
+// Lane exchanges without LDS: DPP moves within rows of 16 lanes, readlane
+// across them.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_i32<CTRL>((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_i32<CTRL>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v of lane l (l uniform)
+__device__ __forceinline__ double lane_f64(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// DPP controls: quad_perm [1,0,3,2] and [2,3,0,1] (lane xor 1, xor 2), then
+// the half-row and row mirrors, which leave every lane with the reduction
+// over its row of 16 lanes; the four rows are then combined by readlane.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+// Maximum of v over the wave (uniform result); v is not NaN.
+__device__ __forceinline__ double wave_max_f64(double v)
+{
+    v = fmax(v, dpp_f64<kDppXor1>(v));
+    v = fmax(v, dpp_f64<kDppXor2>(v));
+    v = fmax(v, dpp_f64<kDppHalfMirror>(v));
+    v = fmax(v, dpp_f64<kDppMirror>(v));
+    return fmax(fmax(lane_f64(v, 0), lane_f64(v, 16)), fmax(lane_f64(v, 32), lane_f64(v, 48)));
+}
+
+// Minimum of v over the wave (uniform result).
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+    v = min(v, dpp_i32<kDppXor1>(v));
+    v = min(v, dpp_i32<kDppXor2>(v));
+    v = min(v, dpp_i32<kDppHalfMirror>(v));
+    v = min(v, dpp_i32<kDppMirror>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// Pivot of position k (LU_Decomposition.cpp:32-49), run by the wave that
+// owns matrix column k with colv = that column brought up to date: the
+// candidate at position i >= k is |a_ik| / s_i (s: the original max-norm
+// by position), the first maximum in position order wins, and a scaled
+// pivot <= 1e-12 is singular.  Publishes in buffer `buf` the exchanged
+// position and the pivot's physical row, the multipliers a_ik / pivot of
+// every physical row still to be eliminated (0 for the others), the pivot
+// value and the singular flag, then raises the buffer's flag to k + 1.
+template <int RB, int CB, int NR, int TC>
+__device__ __forceinline__ void pivot_publish(const double (&colv)[RB], const int (&pos)[RB], const double (&scl)[RB],
+                                              int k, int n, Smem<RB, CB, NR, TC> &s, int buf)
+{
+    const int lane = threadIdx.x % WAVE;
+    double best = -2.0, bval = 0.0;
+    int bpos = 0x7fffffff, brow = 0;
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        const int row = lane + WAVE * a;
+        if (row < n && pos[a] >= k) {
+            double ratio = fabs(colv[a]) / scl[a];
+            if (!(ratio >= 0.0)) ratio = -1.0;  // NaN: lowest priority
+            if (ratio > best || (ratio == best && pos[a] < bpos)) {
+                best = ratio;
+                bpos = pos[a];
+                brow = row;
+                bval = colv[a];
+            }
+        }
+    }
+    const double g = wave_max_f64(best);
+    const bool top = best == g;
+    unsigned long long m = __ballot(top);
+    if (__popcll(m) > 1) {  // several rows share the maximum: the first position wins
+        const int mp = wave_min_i32(top ? bpos : 0x7fffffff);
+        m = __ballot(top && bpos == mp);
+    }
+    const int w = __builtin_ctzll(m);
+    const int ppos = uniform(__builtin_amdgcn_readlane(bpos, w));
+    const int ph = uniform(__builtin_amdgcn_readlane(brow, w));
+    const double piv = lane_f64(bval, w);
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        const int row = lane + WAVE * a;
+        const int np = row == ph ? k : (pos[a] == k ? ppos : pos[a]);
+        s.lbuf[buf][row] = (row < n && np > k) ? colv[a] / piv : 0.0;
+    }
+    if (lane == 0) {
+        s.info[buf][0] = ppos;
+        s.info[buf][1] = ph;
+        s.udiag[k] = piv;
+        s.phys_of_pos[k] = ph;
+        if (g <= kEps) atomicMax(&s.singular, 1);  // includes a NaN column (g = -1)
+        // the wave's multiplier and info stores complete before the flag
+        __hip_atomic_store(&s.flag[buf], k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// Wait until position k is published (its buffer's flag reads k + 1); a
+// wait past kPubSpin trips marks the mode as failed and goes on.
+template <int RB, int CB, int NR, int TC>
+__device__ __forceinline__ void await_position(Smem<RB, CB, NR, TC> &s, int k)
+{
+    const int b = k % kPub;
+    int spins = 0;
+    while (uniform(__hip_atomic_load(&s.flag[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) != k + 1) {
+        if (++spins > kPubSpin) {
+            if (threadIdx.x % WAVE == 0) atomicMax(&s.singular, 2);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Elimination of the mode kernel: the reference's pivot sequence applied to
+// the matrix and the right-hand sides (LUDecomposition and the forward phase
+// of LUSolve).  Exchanges are implicit -- rows keep their registers and only
+// their positions change -- and exact: the reference's a+b-b exchange
+// arithmetic is not reproduced here (a rounding-level difference; the
+// standalone factor kernel keeps it).
+//
+// Position k's pivot is published (multipliers, exchange, flag) by the wave
+// that owns matrix column k.  Every wave runs the positions in order at its
+// own pace, with no work-group barrier: it waits for position k's flag,
+// applies the exchange to its row positions, and -- if it owns column k+1 --
+// first brings that column up to date (the pivot row's entry is read from
+// its lane) and publishes position k+1; then it stages the pivot row's
+// entries of its own columns in LDS (a wave-private broadcast) and updates
+// its columns right of k and the right-hand sides.  The chain from one
+// publication to the next is one column update and one pivot search; the
+// rank-1 updates of the other waves overlap it.  A wave can be at most four
+// positions behind the latest publication (each publication needs its owner
+// to have reached the previous position), so kPub = 8 buffers suffice.
+// Leaves pos, the U rows in A, Y = L^-1 P b, s.udiag, s.phys_of_pos and
+// s.singular.
+template <int RB, int CB, int NR, int TC>
+__device__ __forceinline__ void eliminate_modes(double (&A)[RB][CB], double (&Y)[RB][NR], int (&pos)[RB], int n,
+                                                Smem<RB, CB, NR, TC> &s)
+{
+    const int lane = threadIdx.x % WAVE, c = uniform(threadIdx.x / WAVE);
+    // s by position (positions start as the physical rows), LU_Decomposition.cpp:20-29
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        double m = 0.0;
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+            if (c + TC * b < n) m = fmax(m, fabs(A[a][b]));
+        s.part[c][lane + WAVE * a] = m;
+        pos[a] = lane + WAVE * a;
+    }
+    if (threadIdx.x == 0) s.singular = 0;
+    if (threadIdx.x < kPub) s.flag[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < WAVE * RB; i += WAVE * TC) {
+        double m = s.part[0][i];
+#pragma unroll
+        for (int cc = 1; cc < TC; ++cc) m = fmax(m, s.part[cc][i]);
+        s.scale[i] = m;
+    }
+    __syncthreads();
+    double scl[RB];  // s of each row's current position
+#pragma unroll
+    for (int a = 0; a < RB; ++a) scl[a] = s.scale[lane + WAVE * a];
+    if (n > 1 && c == 0) {
+        double colv[RB];
+#pragma unroll
+        for (int a = 0; a < RB; ++a) colv[a] = A[a][0];
+        pivot_publish<RB, CB, NR, TC>(colv, pos, scl, 0, n, s, 0);
+    }
+
+    for (int k = 0; k < n - 1; ++k) {
+        const int buf = k % kPub;
+        const int ck = k % TC, bk = k / TC;
+        await_position(s, k);
+        const int ppos = uniform(s.info[buf][0]), ph = uniform(s.info[buf][1]);
+        const int pa = ph / WAVE, pl = ph % WAVE;
+        double l[RB];
+#pragma unroll
+        for (int a = 0; a < RB; ++a) l[a] = s.lbuf[buf][lane + WAVE * a];
+        {
+            // the displaced row takes position ppos and its scale, which is
+            // the pivot row's current scale (scl follows the positions)
+            double ps = scl[0];
+#pragma unroll
+            for (int a = 1; a < RB; ++a)
+                if (a == pa) ps = scl[a];
+            ps = lane_f64(ps, pl);
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                const int row = lane + WAVE * a;
+                if (row == ph) {
+                    pos[a] = k;
+                } else if (pos[a] == k) {
+                    pos[a] = ppos;
+                    scl[a] = ps;
+                }
+            }
+        }
+        const int k1 = k + 1;
+        if (k1 < n - 1 && c == k1 % TC) {
+            // look-ahead: column k+1 brought up to date and position k+1's
+            // pivot published before this wave's other work
+            const int b1 = k1 / TC;
+            double colv[RB];
+#pragma unroll
+            for (int a = 0; a < RB; ++a) colv[a] = 0.0;
+#define HZ_COLNEXT(b)                                                           \
+    if constexpr ((b) < CB) {                                                   \
+        _Pragma("unroll") for (int a = 0; a < RB; ++a) colv[a] = A[a][(b)];     \
+    }
+            HZ_SLOT_AT(b1, HZ_COLNEXT)
+#undef HZ_COLNEXT
+            double pown = colv[0];
+#pragma unroll
+            for (int a = 1; a < RB; ++a)
+                if (a == pa) pown = colv[a];
+            const double pv = lane_f64(pown, pl);
+#pragma unroll
+            for (int a = 0; a < RB; ++a) colv[a] = fma(-l[a], pv, colv[a]);
+            pivot_publish<RB, CB, NR, TC>(colv, pos, scl, k1, n, s, k1 % kPub);
+        }
+        // the pivot row's entries of this wave's columns and right-hand sides
+        if (lane == pl) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+                if (a == pa) {
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) s.prow[c][b] = A[a][b];
+#pragma unroll
+                    for (int t = 0; t < NR; ++t) s.prowy[c][t] = Y[a][t];
+                }
+        }
+        wave_sync();
+        // Rank-1 update in chunks of UC slots, each chunk's pivot-row values
+        // loaded together before its FMAs; chunks wholly left of column k are
+        // skipped, and the slots left of k inside the first chunk are updated
+        // too (their rows either have zero multipliers -- rows already
+        // pivoted -- or hold eliminated entries that are never read again).
+        const int b0 = (c > ck) ? bk : bk + 1;  // first slot of this wave right of column k
+        constexpr int UC = CB < 8 ? CB : 8;
+        static_assert(CB % UC == 0, "whole update chunks");
+#pragma unroll
+        for (int ch = 0; ch < CB / UC; ++ch) {
+            if (UC * (ch + 1) > b0) {  // uniform
+                double pv[UC];
+#pragma unroll
+                for (int j = 0; j < UC; ++j) pv[j] = s.prow[c][UC * ch + j];
+#pragma unroll
+                for (int j = 0; j < UC; ++j)
+#pragma unroll
+                    for (int a = 0; a < RB; ++a) A[a][UC * ch + j] = fma(-l[a], pv[j], A[a][UC * ch + j]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+            const double pv = s.prowy[c][t];
+#pragma unroll
+            for (int a = 0; a < RB; ++a) Y[a][t] = fma(-l[a], pv, Y[a][t]);
+        }
+        // the next position's staging rewrites prow: this wave's reads of it are done
+        wave_sync();
+    }
+    __syncthreads();
+    // last position: its row is the last pivot (not tested by the reference)
+    {
+        const int k = n - 1, ck = k % TC, bk = k / TC;
+        if (c == ck) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                const int row = lane + WAVE * a;
+                if (row < n && pos[a] == k) {
+                    double v = 0.0;
+#define HZ_GETV(b)                                                              \
+    if constexpr ((b) < CB) v = A[a][(b)];
+                    HZ_SLOT_AT(bk, HZ_GETV)
+#undef HZ_GETV
+                    s.udiag[k] = v;
+                    s.phys_of_pos[k] = row;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 template <int RB, int CB, int NR, int TC>
 __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
@@ -388,31 +691,40 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
         }
     }
 
-    eliminate<RB, CB, NR, TC>(A, Y, pos, n, nrhs, sm, nullptr, nullptr);
+    eliminate_modes<RB, CB, NR, TC>(A, Y, pos, n, sm);
 
     // ---- back substitution, U staged through LDS in blocks of BQ columns ----
+    // Within a block the positions are unrolled, so the loads of U, of the
+    // pivot rows and of the diagonal issue ahead of the division chain.
     const int nblk = (n + BQ - 1) / BQ;
     int buf = 0;
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int q0 = blk * BQ;
-        // wave c owns the block's columns q0 + c + TC*t, i.e. slots blk*BQ/TC + t
-#define HZ_UWRITE(b)                                                            \
-    if constexpr ((b) < CB) {                                                   \
-        const int j = c + TC * (b);                                             \
-        if (j < q0 + BQ) {                                                      \
-            _Pragma("unroll") for (int aa = 0; aa < RB; ++aa) ubuf[buf][j - q0][lane + WAVE * aa] = A[aa][(b)]; \
+        // wave c owns the block's columns q0 + c + TC*tq, slots blk*BQ/TC + tq
+        switch (blk) {
+#define HZ_UBLK(B)                                                              \
+    case (B):                                                                   \
+        if constexpr (((B) + 1) * (BQ / TC) <= CB) {                            \
+            _Pragma("unroll") for (int tq = 0; tq < BQ / TC; ++tq)              \
+            _Pragma("unroll") for (int aa = 0; aa < RB; ++aa)                   \
+                ubuf[buf][c + TC * tq][lane + WAVE * aa] = A[aa][(B) * (BQ / TC) + tq]; \
         }                                                                       \
-    }
-        {
-            const int bs = blk * (BQ / TC);
-            HZ_SLOTS_FROM(bs, HZ_UWRITE)
+        break;
+            HZ_UBLK(0) HZ_UBLK(1) HZ_UBLK(2) HZ_UBLK(3) HZ_UBLK(4) HZ_UBLK(5) HZ_UBLK(6) HZ_UBLK(7)
+#undef HZ_UBLK
+        default: break;
         }
-#undef HZ_UWRITE
+        static_assert(CB / (BQ / TC) <= 8, "at most 8 back-substitution blocks");
         __syncthreads();
-        const int q_hi = min(n - 1, q0 + BQ - 1);
-        for (int q = q_hi; q >= q0; --q) {
+#pragma unroll
+        for (int qq = BQ - 1; qq >= 0; --qq) {
+            const int q = q0 + qq;
+            if (q >= n) continue;  // uniform
             const int ph = uniform(sm.phys_of_pos[q]);
             const double uqq = sm.udiag[q];
+            double u[RB];
+#pragma unroll
+            for (int aa = 0; aa < RB; ++aa) u[aa] = ubuf[buf][qq][lane + WAVE * aa];
 #pragma unroll
             for (int t = 0; t < NR; ++t) {
                 if (c + TC * t >= nrhs) continue;  // uniform
@@ -420,12 +732,12 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
 #pragma unroll
                 for (int aa = 1; aa < RB; ++aa)
                     if (aa == ph / WAVE) own = Y[aa][t];
-                const double xq = __shfl(own, ph % WAVE) / uqq;
+                const double xq = lane_f64(own, ph % WAVE) / uqq;
 #pragma unroll
                 for (int aa = 0; aa < RB; ++aa) {
                     const int row = lane + WAVE * aa;
                     double v = Y[aa][t];
-                    if (pos[aa] < q && row < n) v = fma(-ubuf[buf][q - q0][row], xq, v);
+                    if (pos[aa] < q && row < n) v = fma(-u[aa], xq, v);
                     if (row == ph) v = xq;
                     Y[aa][t] = v;
                 }
@@ -539,7 +851,7 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
             dst[col + N - 1] = zr;
         }
     }
-    if (threadIdx.x == 0) a.status[mode] = sm.singular ? HZ_ERR_SINGULAR : 0;
+    if (threadIdx.x == 0) a.status[mode] = sm.singular == 2 ? HZ_ERR_HIP : sm.singular ? HZ_ERR_SINGULAR : 0;
 }
 
 // Batched factorisation in the reference's storage (no right-hand sides).

@@ -817,6 +817,8 @@ int hz_solve_status(hz_plan *p, hz_stream_t stream)
     std::vector<int> st(p->modes);
     HZ_HIP(hipMemcpy(st.data(), p->d_status, sizeof(int) * p->modes, hipMemcpyDeviceToHost));
     for (int m = 0; m < p->modes; ++m)
+        if (st[m] == HZ_ERR_HIP) return set_error(HZ_ERR_HIP, "PLAIN_LU: a mode's elimination stalled (wait bound)");
+    for (int m = 0; m < p->modes; ++m)
         if (st[m]) return set_error(HZ_ERR_SINGULAR, "ERROR: Matrix is singular!");
     return HZ_OK;
 }
