@@ -580,7 +580,7 @@ __device__ __forceinline__ void eliminate_modes(double (&A)[RB][CB], double (&Y)
         // too (their rows either have zero multipliers -- rows already
         // pivoted -- or hold eliminated entries that are never read again).
         const int b0 = (c > ck) ? bk : bk + 1;  // first slot of this wave right of column k
-        constexpr int UC = CB < 8 ? CB : 8;
+        constexpr int UC = RB == 2 ? 4 : (CB < 8 ? CB : 8);
         static_assert(CB % UC == 0, "whole update chunks");
 #pragma unroll
         for (int ch = 0; ch < CB / UC; ++ch) {
@@ -626,12 +626,19 @@ __device__ __forceinline__ void eliminate_modes(double (&A)[RB][CB], double (&Y)
     __syncthreads();
 }
 
+// Work-groups per CU the mode kernel is compiled for: three for the
+// 128-row instance with few right-hand sides (168 VGPRs, a few spilled in
+// the assembly phase; measured 14% faster than two at 256^2 x 129), the
+// register file's own limit otherwise.
+template <int RB, int NR>
+constexpr int lu_modes_min_wgs() { return RB == 2 && NR <= 2 ? 3 : 1; }
+
 // ---------------------------------------------------------------------------
 template <int RB, int CB, int NR, int TC>
-__global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
+__global__ __launch_bounds__(WAVE *TC, (lu_modes_min_wgs<RB, NR>())) void lu_modes_kernel(const LuModeArgs a)
 {
     constexpr int ROWS = WAVE * RB;
-    constexpr int BQ = 4 * TC;  // back-substitution block of U columns
+    constexpr int BQ = 2 * TC;  // back-substitution block of U columns
     __shared__ Smem<RB, CB, NR, TC> sm;
     __shared__ double ubuf[2][BQ][ROWS];
     __shared__ double sh_ul[ROWS], sh_ur[ROWS];
@@ -711,10 +718,11 @@ __global__ __launch_bounds__(WAVE *TC) void lu_modes_kernel(const LuModeArgs a)
         }                                                                       \
         break;
             HZ_UBLK(0) HZ_UBLK(1) HZ_UBLK(2) HZ_UBLK(3) HZ_UBLK(4) HZ_UBLK(5) HZ_UBLK(6) HZ_UBLK(7)
+            HZ_UBLK(8) HZ_UBLK(9) HZ_UBLK(10) HZ_UBLK(11) HZ_UBLK(12) HZ_UBLK(13) HZ_UBLK(14) HZ_UBLK(15)
 #undef HZ_UBLK
         default: break;
         }
-        static_assert(CB / (BQ / TC) <= 8, "at most 8 back-substitution blocks");
+        static_assert(CB / (BQ / TC) <= 16, "at most 16 back-substitution blocks");
         __syncthreads();
 #pragma unroll
         for (int qq = BQ - 1; qq >= 0; --qq) {
