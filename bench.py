@@ -31,6 +31,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+
+def progress(msg):
+    """One line per finished part on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
 NX = NY = 256
 N = 129
 LAM = 10.0
@@ -501,6 +506,7 @@ def main():
     results = {}
     for form in args.forms.split(","):
         results[form] = time_form(form, f, gl, gr, args.steps, args.warmup)
+        progress(f"{form}: {results[form]['ms_per_solve']:.4f} ms per solve")
     ok = [r for r in results.values() if r["rel_err_vs_reference"] <= 1e-10]
     chosen = max(ok, key=lambda r: r["solves_per_s"]) if ok else results["plain"]
 
@@ -554,12 +560,18 @@ def main():
     }
     if not args.no_extras and "diag" in results:
         line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
+        progress("adjoint done")
         line["allen_cahn"] = time_allen_cahn()
+        progress("allen_cahn done")
         line["batch"] = time_batch(ceilings=roof["attainable"])
+        progress("batch done")
         line["batch_mixed"] = time_batch(ceilings=roof["attainable"], form="mixed")
+        progress("batch_mixed done")
         line["precision"] = time_precision()
+        progress("precision done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(f, gl, gr)
+        progress("cpu_baseline done")
     print(json.dumps(line))
 
 
