@@ -10,8 +10,10 @@
 //            multiplies by the twiddle W_L^(n2*k1);
 //   exchange through LDS (an L x W tile, lane-fastest, conflict free);
 //   stage 2: R2-point DFTs in registers, stores X[k1 + R1*k2] coalesced.
-// One HBM read and one HBM write per element; the LDS tile is the only
-// transpose.  Forward transforms are unnormalised with exp(-2 pi i jk/L);
+// Lengths 512 and 1024 split stage 2 once more (three register stages, two
+// exchanges through the same tile), which keeps each thread to 8 or 16
+// points.  One HBM read and one HBM write per element; the LDS tile is the
+// only transpose.  Forward transforms are unnormalised with exp(-2 pi i jk/L);
 // inverse ones carry 1/L, as the reference's FourierTransformX/Y
 // (Fourier_Quadrature.cpp:778, :809-817).  R2C keeps points 0..L/2; C2R
 // rebuilds the upper half by conjugate symmetry and keeps the real part.
@@ -22,20 +24,20 @@ namespace {
 
 // Uniform-base tiles (fft_tile_uniform): one 1-D grid of main-field tiles
 // then extra-field tiles.
-template <int L, int R1, int R2, int W, int KIND>
-__global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines(const FftArgs a)
+template <int L, int R1, int R2, int W, int KIND, int RB>
+__global__ __launch_bounds__((W * fft_threads_per_line<R1, R2, RB>())) void fft_lines(const FftArgs a)
 {
     __shared__ double tile[(L + 2) * W];
-    fft_tile_uniform<L, R1, R2, W, KIND>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
+    fft_tile_uniform<L, R1, R2, W, KIND, RB>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
 }
 
 // Per-thread addressing, for launches whose offsets do not fit 32 bits.
-template <int L, int R1, int R2, int W, int KIND>
-__global__ __launch_bounds__(W *(R1 > R2 ? R1 : R2)) void fft_lines_wide(const FftArgs a)
+template <int L, int R1, int R2, int W, int KIND, int RB>
+__global__ __launch_bounds__((W * fft_threads_per_line<R1, R2, RB>())) void fft_lines_wide(const FftArgs a)
 {
     __shared__ double tile[(L + 2) * W];
     const int nbx = gridDim.x, logical = xcd_logical(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
-    fft_tile<L, R1, R2, W, KIND>(a, logical % nbx, logical / nbx, tile);
+    fft_tile<L, R1, R2, W, KIND, RB>(a, logical % nbx, logical / nbx, tile);
 }
 
 // Largest element index a uniform-base tile of this launch forms, on either
@@ -53,31 +55,32 @@ bool offsets_fit_32(int L, int W, const FftArgs &a)
     return m_in < lim && m_out < lim && x_in < lim && x_out < lim && m_in >= 0 && m_out >= 0;
 }
 
-template <int L, int R1, int R2, int W>
+// RB > 1: the three-stage core (stage 2 split into R2 / RB and RB).
+template <int L, int R1, int R2, int W, int RB = 1>
 int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
 {
-    constexpr int Q = R1 > R2 ? R1 : R2;
+    constexpr int Q = fft_threads_per_line<R1, R2, RB>();
     const dim3 block(W * Q);
     if (offsets_fit_32(L, W, a)) {
         const long long tiles = uniform_main_tiles(a.nbatch, a.lanes, a.in_batch_stride, a.out_batch_stride, W) +
                                 (long long)a.xlanes * ((a.nbatch + W - 1) / W);
         const dim3 grid((unsigned)tiles);
         switch (kind) {
-        case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
-        case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
-        case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a); break;
-        case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a); break;
+        case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a); break;
+        case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R, RB>), grid, block, 0, s, a); break;
+        case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD, RB>), grid, block, 0, s, a); break;
+        case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV, RB>), grid, block, 0, s, a); break;
         }
     } else {
         const dim3 grid((a.lanes + a.xlanes + W - 1) / W, a.nbatch);
         switch (kind) {
-        case FFT_R2C: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C>), grid, block, 0, s, a); break;
-        case FFT_C2R: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2R>), grid, block, 0, s, a); break;
+        case FFT_R2C: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a); break;
+        case FFT_C2R: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2R, RB>), grid, block, 0, s, a); break;
         case FFT_C2C_FWD:
-            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_FWD>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_FWD, RB>), grid, block, 0, s, a);
             break;
         case FFT_C2C_INV:
-            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_INV>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_INV, RB>), grid, block, 0, s, a);
             break;
         }
     }
@@ -144,8 +147,10 @@ int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
     case 256:
         return a.w == 8 ? launch_kind<256, 16, 16, 8>(kind, a, s)
                : a.w == 32 ? launch_kind<256, 16, 16, 32>(kind, a, s) : launch_kind<256, 16, 16, 16>(kind, a, s);
-    case 512: return launch_kind<512, 32, 16, 8>(kind, a, s);
-    case 1024: return launch_kind<1024, 32, 32, 4>(kind, a, s);
+    // three stages (fft_core3): 8 x 8 x 8 and 16 x 8 x 8, 59-66 and 110-114
+    // VGPRs (the two-stage 32 x 16 and 32 x 32 splits held 206-256)
+    case 512: return launch_kind<512, 8, 64, 16, 8>(kind, a, s);
+    case 1024: return launch_kind<1024, 16, 64, 8, 8>(kind, a, s);
     default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [1, 1024]");
     }
 }

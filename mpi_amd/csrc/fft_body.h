@@ -263,10 +263,220 @@ __device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *
     }
 }
 
+// W_L^(e * k) for k < R from per-digit powers: pw[m][d] = (W_L^(e 4^m))^d,
+// two table loads per digit position and at most four roundings.
+template <int R, int L>
+struct TwiddlePowers {
+    static constexpr int ND = (ilog2(R) + 1) / 2;
+    double2 pw[ND][4];
+    __device__ __forceinline__ void load(const double2 *__restrict__ tw, int e)
+    {
+#pragma unroll
+        for (int m = 0; m < ND; ++m) {
+            const double2 b = tw[(e << (2 * m)) % L];
+            pw[m][1] = b;
+            pw[m][2] = cmul(b, b);
+            pw[m][3] = cmul(pw[m][2], b);
+        }
+    }
+    // v[k] *= W^(e k) (SIGN applied to the imaginary part)
+    template <int SIGN>
+    __device__ __forceinline__ void apply(double (&re)[R], double (&im)[R]) const
+    {
+#pragma unroll
+        for (int k = 1; k < R; ++k) {
+            double2 t = make_double2(1.0, 0.0);
+            bool first = true;
+#pragma unroll
+            for (int m = 0; m < ND; ++m) {
+                const int d = (k >> (2 * m)) & 3;
+                if (d == 0) continue;
+                t = first ? pw[m][d] : cmul(t, pw[m][d]);
+                first = false;
+            }
+            const double vr = re[k], vi = im[k];
+            const double wr = t.x, wi = SIGN * t.y;
+            re[k] = vr * wr - vi * wi;
+            im[k] = vr * wi + vi * wr;
+        }
+    }
+};
+
+// Three-stage form of fft_core for long lines (L = R1 * RA * RB, M = RA RB):
+//   stage 1: thread q < M: R1-point DFT of x[q + M n1], twiddle W_L^(q k1);
+//   stage 2: thread (k1, t), t < RB: RA-point DFT over a of the stage-1
+//            outputs y_k1[t + RB a], twiddle W_M^(t ka);
+//   stage 3: thread (k1, ka): RB-point DFT over t of z_k1[ka RB + t],
+//            X[k1 + R1 (ka + RA kb)].
+// (W_M^(q kM) with q = t + RB a, kM = ka + RA kb factors into
+// W_RA^(a ka) W_M^(t ka) W_RB^(t kb).)  Each thread holds at most
+// max(R1, RA, RB) points, so the registers stay far below the two-stage
+// split's for L = 512, 1024; the exchanges go through the same tile, real
+// parts then imaginary parts.  Same loads, stores and scaling as fft_core.
+template <int L, int R1, int RA, int RB, int W, int KIND, class A>
+__device__ __forceinline__ void fft_core3(const A &ad, bool live, const double2 *__restrict__ tw, double *tile)
+{
+    constexpr int M = RA * RB;
+    static_assert(R1 * M == L, "radix split");
+    constexpr int SIGN = (KIND == FFT_R2C || KIND == FFT_C2C_FWD) ? -1 : 1;
+    __syncthreads();
+    const int w = threadIdx.x % W, q = threadIdx.x / W;
+
+    // ---- stage 1 ----
+    double sre[R1], sim[R1];
+    if constexpr (KIND == FFT_C2R) {
+        constexpr int H = L / 2 + 1, Q = M > R1 * RB ? (M > R1 * RA ? M : R1 * RA) : (R1 * RB > R1 * RA ? R1 * RB : R1 * RA);
+        constexpr int NJ = (H + Q - 1) / Q;
+        double hr[NJ], hi[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int pt = q + Q * j;
+            hr[j] = hi[j] = 0.0;
+            if (live && pt < H) {
+                hr[j] = ad.ldr(pt);
+                hi[j] = ad.ldi(pt);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int pt = q + Q * j;
+            if (pt < H) {
+                tile[pt * W + w] = hr[j];
+                tile[(H + pt) * W + w] = hi[j];
+            }
+        }
+        __syncthreads();
+        if (q < M) {
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) {
+                const int p = q + M * n1;
+                const int pp = p <= L / 2 ? p : L - p;
+                sre[n1] = tile[pp * W + w];
+                const double vi = tile[(H + pp) * W + w];
+                sim[n1] = p > L / 2 ? -vi : vi;
+            }
+        }
+        __syncthreads();
+    } else if (q < M) {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+            const int p = q + M * n1;
+            double vr = 0.0, vi = 0.0;
+            if (live) {
+                vr = ad.ldr(p);
+                if (KIND != FFT_R2C) vi = ad.ldi(p);
+            }
+            sre[n1] = vr;
+            sim[n1] = vi;
+        }
+    }
+    if (q < M) {
+        TwiddlePowers<R1, L> t1;
+        t1.load(tw, q);
+        dft_regs<R1, L, SIGN>(sre, sim, tw);
+        t1.template apply<SIGN>(sre, sim);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) tile[(k1 * M + q) * W + w] = sre[k1];
+    }
+    __syncthreads();
+    // ---- stage 2: thread (k1, t) ----
+    const int k1b = q / RB, tb = q % RB;
+    const bool s2 = q < R1 * RB;
+    double ar[RA], ai[RA];
+    if (s2) {
+#pragma unroll
+        for (int a = 0; a < RA; ++a) ar[a] = tile[(k1b * M + tb + RB * a) * W + w];
+    }
+    __syncthreads();
+    if (q < M) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) tile[(k1 * M + q) * W + w] = sim[k1];
+    }
+    __syncthreads();
+    if (s2) {
+#pragma unroll
+        for (int a = 0; a < RA; ++a) ai[a] = tile[(k1b * M + tb + RB * a) * W + w];
+        TwiddlePowers<RA, L> t2;
+        t2.load(tw, R1 * tb);  // W_M^(t ka) = W_L^(R1 t ka)
+        dft_regs<RA, L, SIGN>(ar, ai, tw);
+        t2.template apply<SIGN>(ar, ai);
+    }
+    __syncthreads();
+    if (s2) {
+#pragma unroll
+        for (int ka = 0; ka < RA; ++ka) tile[(k1b * M + ka * RB + tb) * W + w] = ar[ka];
+    }
+    __syncthreads();
+    // ---- stage 3: thread (k1, ka) ----
+    const int k1c = q / RA, kac = q % RA;
+    const bool s3 = q < R1 * RA;
+    double br[RB], bi[RB];
+    if (s3) {
+#pragma unroll
+        for (int t = 0; t < RB; ++t) br[t] = tile[(k1c * M + kac * RB + t) * W + w];
+    }
+    __syncthreads();
+    if (s2) {
+#pragma unroll
+        for (int ka = 0; ka < RA; ++ka) tile[(k1b * M + ka * RB + tb) * W + w] = ai[ka];
+    }
+    __syncthreads();
+    if (s3) {
+#pragma unroll
+        for (int t = 0; t < RB; ++t) bi[t] = tile[(k1c * M + kac * RB + t) * W + w];
+        dft_regs<RB, L, SIGN>(br, bi, tw);
+        if (live) {
+            constexpr double scale = 1.0 / L;
+#pragma unroll
+            for (int kb = 0; kb < RB; ++kb) {
+                const int kk = k1c + R1 * (kac + RA * kb);
+                if (KIND == FFT_R2C) {
+                    if (kk <= L / 2) {
+                        ad.str(kk, br[kb]);
+                        ad.sti(kk, bi[kb]);
+                    }
+                } else if (KIND == FFT_C2R) {
+                    ad.str(kk, br[kb] * scale);
+                } else if (KIND == FFT_C2C_INV) {
+                    ad.str(kk, br[kb] * scale);
+                    ad.sti(kk, bi[kb] * scale);
+                } else {
+                    ad.str(kk, br[kb]);
+                    ad.sti(kk, bi[kb]);
+                }
+            }
+        }
+    }
+}
+
+// Threads per line of a transform: max(R1, R2) for the two-stage split
+// (R2 points per stage-2 thread), the largest stage for the three-stage one
+// (R2 = RA * RB split into RB > 1).
+template <int R1, int R2, int RB>
+constexpr int fft_threads_per_line()
+{
+    if constexpr (RB == 1) {
+        return R1 > R2 ? R1 : R2;
+    } else {
+        constexpr int RA = R2 / RB, a = R2, b = R1 * RB, c = R1 * RA;
+        return a > b ? (a > c ? a : c) : (b > c ? b : c);
+    }
+}
+
+// The two- or three-stage core (RB > 1 splits stage 2 into RA = R2 / RB and RB).
+template <int L, int R1, int R2, int RB, int W, int KIND, class A>
+__device__ __forceinline__ void fft_core_any(const A &ad, bool live, const double2 *__restrict__ tw, double *tile)
+{
+    if constexpr (RB == 1)
+        fft_core<L, R1, R2, W, KIND>(ad, live, tw, tile);
+    else
+        fft_core3<L, R1, R2 / RB, RB, W, KIND>(ad, live, tw, tile);
+}
+
 // One (lane block, line batch) tile, lanes = the main field's z lanes then
 // the extra single-lane fields (faces), per-thread addressing (the batch
 // kernels' tile).
-template <int L, int R1, int R2, int W, int KIND>
+template <int L, int R1, int R2, int W, int KIND, int RB = 1>
 __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int batch, double *tile)
 {
     const int w = threadIdx.x % W;
@@ -285,7 +495,7 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
     ad.ips = xl ? sval(a.x_in_point_stride) : sval(a.in_point_stride);
     ad.ob = xl ? (long long)batch * sval(a.x_out_batch_stride) : (long long)batch * sval(a.out_batch_stride) + lane;
     ad.ops = xl ? sval(a.x_out_point_stride) : sval(a.out_point_stride);
-    fft_core<L, R1, R2, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
+    fft_core_any<L, R1, R2, RB, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
 }
 
 // Line batches whose lanes follow each other in memory on both sides (batch
@@ -312,7 +522,7 @@ __host__ __device__ inline long long uniform_main_tiles(int nbatch, int lanes, l
 // when lanes_merge holds); the rest are the extra fields,
 // xlanes * ceil(nbatch / W) tiles whose W "lanes" are W consecutive line
 // batches of one extra field.
-template <int L, int R1, int R2, int W, int KIND>
+template <int L, int R1, int R2, int W, int KIND, int RB = 1>
 __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double *tile)
 {
     const int w = threadIdx.x % W;
@@ -353,7 +563,7 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ops = (unsigned)a.x_out_point_stride;
         live = b0 + w < nb;
     }
-    fft_core<L, R1, R2, W, KIND>(ad, live, a.tw, tile);
+    fft_core_any<L, R1, R2, RB, W, KIND>(ad, live, a.tw, tile);
 }
 
 // Element t of a length-1 line pass (an axis of one point: the 2-D and 1-D
