@@ -213,11 +213,13 @@ int hz_lu_solve_batched(const double *LU, const int *piv, double *b, int n, int 
  * Two ways to run a batch, each with mode-solve tiles of 32 or 64 rows:
  * staged (variants 4, 5: one launch per stage and kind of task, recorded in
  * a HIP graph that later solves replay, re-reading the buffer table when the
- * pointers change) and fused (variants 6, 7: ONE cooperative launch per
- * solve, the five stages separated by grid barriers; built but switched off
- * and refused, since two of its GPU runs did not finish).  The first solve of
- * a batch shape times every usable variant and keeps the fastest.  Variants
- * 0-3 are refused (an earlier fused design, removed).
+ * pointers change), merged (variants 8, 9: one launch per stage over every
+ * task kind of the stage, also recorded in a graph) and fused (variants 6, 7:
+ * ONE cooperative launch per solve, the five stages separated by grid
+ * barriers; built but switched off and refused, since two of its GPU runs did
+ * not finish).  The first solve of a batch shape times every usable variant
+ * and keeps the fastest.  Variants 0-3 are refused (an earlier fused design,
+ * removed).
  * Enqueued on `stream`; a plan in a batch must not be used by another solve
  * while the batch runs. */
 typedef struct hz_batch hz_batch;
@@ -227,13 +229,14 @@ int hz_batch_solve(hz_batch *batch, const double *const *f, const double *const 
 /* Synchronises the batch and reports HZ_ERR_HIP if the fused kernel ran
  * without a cooperative launch (it then does no work). */
 int hz_batch_status(hz_batch *batch);
-/* The autotuner's choice (variant index, fused or staged, tile rows, the
- * fused launch's work-groups per CU) and the measured ms of every variant
- * (0 where not timed). */
+/* The autotuner's choice (variant index; fused = 0 staged per task kind,
+ * 1 fused, 2 merged per stage; tile rows; the fused launch's work-groups per
+ * CU) and the measured ms of every variant (0 where not timed). */
 int hz_batch_tuning(const hz_batch *batch, int *variant, int *fused, int *tile_rows,
                     int *wgs_per_cu, double *variant_ms, int max_variants);
-/* Pins the variant (4 or 5; 0-3 and 6-7 are refused; -1 = autotune again on
- * the next solve); drops the recorded graph.  For tests and tuning studies. */
+/* Pins the variant (4, 5, 8 or 9; 0-3 and 6-7 are refused; -1 = autotune
+ * again on the next solve); drops the recorded graph.  For tests and tuning
+ * studies. */
 int hz_batch_force_variant(hz_batch *batch, int variant);
 int hz_batch_destroy(hz_batch *batch);
 /* One-shot form of the above: create the batch, solve, wait for it,

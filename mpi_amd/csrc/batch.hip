@@ -196,6 +196,71 @@ __global__ __launch_bounds__(256, 2) void batch_coop_kernel(const BatchProblem *
     }
 }
 
+// Merged launches: one launch per stage over every task kind of the stage
+// (the per-XCD order of the fused variant, XCD-aware work-group mapping), the
+// task bodies inlined behind a switch, so small kinds share a launch with
+// large ones.  No waits: the launch boundaries order the stages.
+template <int KIND>
+__device__ __forceinline__ void merged_fft(const FftArgs &a, int L, int t, double *smem)
+{
+    if (L == 1) {
+        unit_line_element<KIND>(a, (long long)t * 256 + threadIdx.x);
+        return;
+    }
+    const int W = L <= 16 ? 64 : L <= 64 ? 32 : 16;
+    const long long tiles = uniform_main_tiles(a.nbatch, a.lanes, a.in_batch_stride, a.out_batch_stride, W) +
+                            (long long)a.xlanes * ((a.nbatch + W - 1) / W);
+    if (t >= tiles) return;
+    switch (L) {
+    case 16: fft_tile_uniform<16, 4, 4, 64, KIND>(a, t, smem); break;
+    case 32: fft_tile_uniform<32, 8, 4, 32, KIND>(a, t, smem); break;
+    case 64: fft_tile_uniform<64, 8, 8, 32, KIND>(a, t, smem); break;
+    case 128: fft_tile_uniform<128, 16, 8, 16, KIND>(a, t, smem); break;
+    default: fft_tile_uniform<256, 16, 16, 16, KIND>(a, t, smem); break;
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void batch_stage_fft(const BatchProblem *__restrict__ P,
+                                                       const int2 *__restrict__ list, int pass)
+{
+    extern __shared__ double smem[];
+    const int2 e = list[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchProblem &pr = P[e.x];
+    merged_fft<KIND>(pr.f[pass], pass == 0 || pass == 3 ? pr.ly : pr.lx, e.y, smem);
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void batch_stage_modes(const BatchProblem *__restrict__ P,
+                                                         const int2 *__restrict__ list)
+{
+    extern __shared__ double smem[];
+    const int2 e = list[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchProblem &pr = P[e.x];
+    switch (pr.dkind) {
+    case 0: coop_diag_t<2, RT, false>(pr.d, e.y, smem); break;
+    case 1: coop_diag_t<2, RT, true>(pr.d, e.y, smem); break;
+    case 2: coop_diag_t<4, RT, false>(pr.d, e.y, smem); break;
+    case 3: coop_diag_t<4, RT, true>(pr.d, e.y, smem); break;
+    case 4: coop_diag_t<6, RT, false>(pr.d, e.y, smem); break;
+    case 5: coop_diag_t<6, RT, true>(pr.d, e.y, smem); break;
+    case 6: coop_diag_t<8, RT, false>(pr.d, e.y, smem); break;
+    case 7: coop_diag_t<8, RT, true>(pr.d, e.y, smem); break;
+    case 8: coop_mixed_t<2, 32, false>(pr.m, e.y, smem); break;
+    case 9: coop_mixed_t<2, 32, true>(pr.m, e.y, smem); break;
+    case 10: coop_mixed_t<4, 16, false>(pr.m, e.y, smem); break;
+    case 11: coop_mixed_t<4, 16, true>(pr.m, e.y, smem); break;
+    case 12: coop_mixed_t<6, 32, false>(pr.m, e.y, smem); break;
+    case 13: coop_mixed_t<6, 32, true>(pr.m, e.y, smem); break;
+    case 14: coop_mixed_t<8, 16, false>(pr.m, e.y, smem); break;
+    default: coop_mixed_t<8, 16, true>(pr.m, e.y, smem); break;
+    }
+}
+
+// Dynamic LDS of the merged transform launches: the largest transform tile
+// (the mode-solve launches take coop_lds).
+constexpr size_t merged_fft_lds() { return sizeof(double) * (256 + 2) * 16; }
+
 // Staged launches: one launch per stage and kind of task (transform length,
 // or mode-solve variant), each work-group one task from a (problem, task)
 // list; the launch boundaries order the stages, and every kernel inlines only
@@ -254,7 +319,8 @@ namespace {
 
 struct Variant {
     int rt;           // diag tile rows
-    int fused;        // 1 one launch for the whole batch, 0 one launch per stage and task kind
+    int fused;        // 1 one launch for the whole batch, 0 one launch per stage and task kind,
+                      // 2 one launch per stage over every task kind
     const char *off;  // why the variant is refused, or null: offered to the autotuner and to hz_batch_force_variant
 };
 
@@ -270,7 +336,8 @@ constexpr const char *kRemoved = "batch: variants 0-3 were removed";
 constexpr const char *kCoopOff = "batch: the fused cooperative variants 6 and 7 are switched off: two GPU runs did "
                                  "not finish and the cause is not established";
 const Variant kVariants[] = {{32, 1, kRemoved}, {32, 1, kRemoved}, {32, 1, kRemoved}, {64, 1, kRemoved},
-                             {32, 0, nullptr},  {64, 0, nullptr},  {32, 1, kCoopOff}, {64, 1, kCoopOff}};
+                             {32, 0, nullptr},  {64, 0, nullptr},  {32, 1, kCoopOff}, {64, 1, kCoopOff},
+                             {32, 2, nullptr},  {64, 2, nullptr}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 std::mutex g_tune_mu;
@@ -293,6 +360,7 @@ struct hz_batch {
     // work-groups per CU (0: the device cannot launch cooperatively)
     int2 *d_coop_list[2] = {nullptr, nullptr};
     int *d_coop_off[2] = {nullptr, nullptr};
+    std::vector<int> h_coop_off[2];  // host copy (the merged variant's launch sizes)
     int *d_ctr = nullptr;
     int coop_wgs[2] = {0, 0};
     std::vector<const void *> last;
@@ -404,6 +472,28 @@ template <int RT>
 int enqueue_t(hz_batch *b, const Variant &v, hipStream_t s)
 {
     const int r = rt_index(RT);
+    if (v.fused == 2) {
+        const auto &xo = b->h_coop_off[r];
+        for (int st = 0; st < kStages; ++st) {
+            const int first = xo[(size_t)st * (kXcds + 1)], count = xo[(size_t)st * (kXcds + 1) + kXcds] - first;
+            if (count == 0) continue;
+            const int2 *list = b->d_coop_list[r] + first;
+            const dim3 g(count), blk(256);
+            const int pass = st < 2 ? st : st - 1;
+            const unsigned fl = (unsigned)merged_fft_lds();
+            switch (st) {
+            case 0: hipLaunchKernelGGL((batch_stage_fft<FFT_R2C>), g, blk, fl, s, b->d_probs, list, pass); break;
+            case 1: hipLaunchKernelGGL((batch_stage_fft<FFT_C2C_FWD>), g, blk, fl, s, b->d_probs, list, pass); break;
+            case 2:
+                hipLaunchKernelGGL((batch_stage_modes<RT>), g, blk, (unsigned)coop_lds<RT>(), s, b->d_probs, list);
+                break;
+            case 3: hipLaunchKernelGGL((batch_stage_fft<FFT_C2C_INV>), g, blk, fl, s, b->d_probs, list, pass); break;
+            default: hipLaunchKernelGGL((batch_stage_fft<FFT_C2R>), g, blk, fl, s, b->d_probs, list, pass); break;
+            }
+        }
+        HZ_HIP(hipGetLastError());
+        return HZ_OK;
+    }
     if (v.fused) {
         if (b->coop_wgs[r] < 1) return set_error(HZ_ERR_UNSUPPORTED, "batch: no cooperative launch on this device");
         const BatchProblem *P = b->d_probs;
@@ -508,6 +598,7 @@ int build_lists(hz_batch *b, int rt)
         return set_error(HZ_ERR_ALLOC, "batch allocation failed");
     HZ_HIP(hipMemcpy(b->d_coop_list[r], co.data(), sizeof(int2) * co.size(), hipMemcpyHostToDevice));
     HZ_HIP(hipMemcpy(b->d_coop_off[r], xo.data(), sizeof(int) * xo.size(), hipMemcpyHostToDevice));
+    b->h_coop_off[r] = xo;
     return HZ_OK;
 }
 
@@ -581,7 +672,7 @@ int coop_residency(int *wgs)
 bool variant_usable(const hz_batch *b, int vi)
 {
     const Variant &v = kVariants[vi];
-    return !v.off && (!v.fused || b->coop_wgs[rt_index(v.rt)] > 0);
+    return !v.off && (v.fused != 1 || b->coop_wgs[rt_index(v.rt)] > 0);
 }
 
 void free_batch(hz_batch *b)
@@ -654,6 +745,12 @@ int hz_batch_create(hz_plan *const *plans, int count, hz_batch **batch)
     if (hipMemset(b->d_ctl, 0, sizeof(int) * 4) != hipSuccess ||
         hipMemset(b->d_ctr, 0, sizeof(int) * kStages * kXcds) != hipSuccess)
         return fail(set_error(HZ_ERR_HIP, "batch control clear failed"));
+    // the merged mode-solve launch of 64-row tiles takes more than 64 KiB of dynamic LDS
+    if (hipFuncSetAttribute((const void *)batch_stage_modes<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)coop_lds<64>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)batch_stage_modes<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)coop_lds<32>()) != hipSuccess)
+        return fail(set_error(HZ_ERR_HIP, "batch: cannot set the merged launch's LDS size"));
     int coop = 0;
     if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess) coop = 0;
     if (coop) {
@@ -729,7 +826,7 @@ int hz_batch_solve(hz_batch *b, const double *const *f, const double *const *gl,
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[b->shape] = bi;
     }
-    if (kVariants[b->variant].fused) {
+    if (kVariants[b->variant].fused == 1) {
         // one cooperative launch per solve (not recorded in a graph)
         if ((st = enqueue(b, b->variant, b->stream))) return st;
         HZ_HIP(hipEventRecord(b->ev_out, b->stream));

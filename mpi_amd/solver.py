@@ -228,12 +228,12 @@ class HelmholtzPlan:
 
 class BatchSolver:
     """Many DIAG- or MIXED-form plans solved together (hz_batch_*): one launch
-    per stage and task kind (replayed from a HIP graph) or one cooperative
-    fused launch for the whole batch, chosen by the autotuner on the first
-    solve.  Variants: 4, 5 staged (mode-solve tiles of 32, 64 rows); 6, 7
-    fused (switched off) and 0-3 refused."""
+    per stage and task kind, or one launch per stage over every kind (both
+    replayed from a HIP graph), chosen by the autotuner on the first solve.
+    Variants: 4, 5 staged and 8, 9 merged (mode-solve tiles of 32, 64 rows);
+    6, 7 fused (switched off) and 0-3 refused."""
 
-    NUM_VARIANTS = 8
+    NUM_VARIANTS = 10
 
     def __init__(self, plans):
         self.lib = load()
@@ -270,7 +270,8 @@ class BatchSolver:
         check(self.lib.hz_batch_tuning(self.handle, ctypes.byref(v), ctypes.byref(fused), ctypes.byref(rt),
                                        ctypes.byref(wgs), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                        self.NUM_VARIANTS))
-        return {"variant": v.value, "fused": bool(fused.value), "tile_rows": rt.value,
+        return {"variant": v.value, "launches": ("per stage and kind", "fused", "per stage")[fused.value],
+                "tile_rows": rt.value,
                 "workgroups_per_cu": wgs.value, "variant_ms": ms.tolist()}
 
     def close(self):

@@ -1,6 +1,6 @@
 """Many problems of uneven size in one batch (hz_batch_*).
 
-Every variant (per-stage launches or one cooperative fused launch, both
+Every usable variant (launches per stage and task kind, or per stage over every kind, both
 tile heights) must reproduce each problem's single-plan DIAG solve, which is
 itself pinned to the oracle; a few problems are also checked against the
 oracle directly.  Bar: relative max-norm difference <= 1e-13 against the
@@ -60,11 +60,12 @@ FUSED_OFF = ("variants 0-3, a persistent launch whose tasks waited on per-proble
 # cause is not established.
 COOP_OFF = ("fused cooperative launch (variants 6, 7) switched off: two GPU runs did not finish; cause not "
             "established")
-STAGED = (4, 5)
+STAGED, MERGED = (4, 5), (8, 9)
 
 
 @pytest.mark.parametrize("variant", [pytest.param(v, marks=pytest.mark.skip(reason=FUSED_OFF)) for v in range(4)]
-                         + list(STAGED) + [pytest.param(v, marks=pytest.mark.skip(reason=COOP_OFF)) for v in (6, 7)])
+                         + list(STAGED) + [pytest.param(v, marks=pytest.mark.skip(reason=COOP_OFF)) for v in (6, 7)]
+                         + list(MERGED))
 def test_batch_variant_matches_single_solves(cuda, variant):
     from mpi_amd.solver import BatchSolver
     plans, inputs, _ = make_set(MIXED_SET)
@@ -151,10 +152,10 @@ def test_batch_many_problem_workload(cuda):
         worst = max(worst, rel(u.cpu().numpy(), p.solve(f, gl, gr).cpu().numpy()))
     print(f"worst relative difference over 512 problems: {worst:.3e}")
     assert worst <= 1e-13
-    # both staged variants give the same bits over the 512 problems
+    # the staged and merged variants give the same bits over the 512 problems
     import torch
     first = [u.clone() for u in us]
-    for v in STAGED:
+    for v in STAGED + MERGED:
         b.force_variant(v)
         for rep in range(2):
             got = b.solve(fs, gls, grs)
@@ -174,7 +175,7 @@ def test_batch_rejects_bad_sets(cuda):
     with pytest.raises(HzError):
         BatchSolver([p, q])
     b = BatchSolver([p])
-    for v in (0, 3, 6, 7, 8, -2):  # 0-3 removed, 6-7 switched off; 8, -2 out of range
+    for v in (0, 3, 6, 7, 10, -2):  # 0-3 removed, 6-7 switched off; 10, -2 out of range
         with pytest.raises(HzError):
             b.force_variant(v)
     b.close()
@@ -233,6 +234,12 @@ def test_batch_with_periodic_problems(cuda):
     b.status()
     for k, (u, w) in enumerate(zip(us, want)):
         assert rel(u.cpu().numpy(), w) <= 1e-13, f"problem {k}"
+    for v in MERGED:  # periodic tiles (no faces) in the merged launches
+        b.force_variant(v)
+        got = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(got, want)):
+            assert rel(u.cpu().numpy(), w) <= 1e-13, f"merged variant {v}, problem {k}"
     for k, d in enumerate(pdescs):
         # Bar: 1e-12, or twice the reference algorithm's own rounding spread
         # (the oracle built with FMA contraction) where that is larger: the
@@ -273,6 +280,12 @@ def test_batch_with_two_and_one_dimensional_problems(cuda):
         b.status()
         for k, (u, w) in enumerate(zip(us, want)):
             assert rel(u.cpu().numpy(), w) <= 1e-13, f"solve {rep}, problem {k} {specs[k]}"
+    for v in MERGED:  # length-1 passes in the merged launches
+        b.force_variant(v)
+        us = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(us, want)):
+            assert rel(u.cpu().numpy(), w) <= 1e-13, f"merged variant {v}, problem {k} {specs[k]}"
     b.close()
 
 
@@ -326,6 +339,13 @@ def test_batch_with_mixed_plans(cuda):
         for k, (u, w) in enumerate(zip(us, want)):
             e = rel(u.cpu().numpy(), w)
             assert e <= 1e-13, f"rep {rep} problem {k} ({forms[k]}): {e:.3e}"
+    for v in MERGED:  # MIXED tiles in the merged launches
+        b.force_variant(v)
+        got = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(got, want)):
+            e = rel(u.cpu().numpy(), w)
+            assert e <= 1e-13, f"merged variant {v} problem {k} ({forms[k]}): {e:.3e}"
     for k in (0, 2, 3, 6):
         f, gl, gr = inputs[k]
         uo = po.solve(descs[k], f, gl, gr)
