@@ -284,10 +284,21 @@ int hz_mixed_fp64_modes(const hz_plan *plan, int *fp64_modes);
  * hz_host_plan_operators: GLL nodes on [-1,1] (x[n]), the z array
  * (z[ne*(n-1)+1]) and the scaled Dz, Dzz (n*n each); any pointer may be NULL.
  * hz_host_eigen: Dzz_int = V diag(mu) V^-1 for the (n-2)x(n-2) interior
- * operator of the same plan (row-major V, Vinv; mu ascending). */
+ * operator of the same plan (row-major V, Vinv; mu ascending).
+ * hz_host_eigen_split: the even/odd eigenbasis that the DIAG and adjoint
+ * kernels use.  With m = n-2 and the fold
+ *   F x = (x_i + x_{m-1-i}, i < m/2 [+ the centre x_{m/2} when m is odd] |
+ *          x_i - x_{m-1-i}, i < m/2)
+ * (even coordinates at 0.., odd ones at np/2..), the symmetrised operator
+ * (A + JAJ)/2, J the reversal, equals F^-1 W diag(mu) Winv F with W, Winv
+ * block diagonal (np x np, row-major, zero padding) and mu[np] (zero
+ * padding); np even with np/2 >= ceil(m/2).  *asym = max|A - JAJ| / max|A|
+ * (the rounding of the GLL nodes).  Pointers may be NULL. */
 int hz_host_plan_operators(int n, int ne, double lz, double *x, double *z, double *Dz,
                            double *Dzz);
 int hz_host_eigen(int n, int ne, double lz, double *mu, double *V, double *Vinv);
+int hz_host_eigen_split(int n, int ne, double lz, int np, double *mu, double *W, double *Winv,
+                        double *asym);
 
 /* Stage timing with HIP events on the solve stream (measurement hook).
  * While enabled, every solve records an event before and after each of its

@@ -74,6 +74,7 @@ SIGNATURES = {
     "hz_plan_set_diag_tile": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),
     "hz_host_eigen": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD]),
+    "hz_host_eigen_split": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _PD, _PD, _PD, _PD]),
     "hz_plan_profile": (ctypes.c_int, [_VP, ctypes.c_int]),
     "hz_plan_profile_read": (ctypes.c_int, [_VP, _PD, ctypes.c_int, _PI, _PI, ctypes.c_char_p, ctypes.c_int]),
     "hz_last_error_string": (ctypes.c_char_p, []),

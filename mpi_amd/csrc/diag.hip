@@ -5,11 +5,11 @@
 namespace hz {
 namespace {
 
-template <int NJ, int RT, bool GENERAL, bool ADJ>
+template <int NJ, int RT, bool GENERAL, bool ADJ, bool EO>
 __global__ __launch_bounds__(256, RT >= 128 ? 1 : 2) void diag_modes_kernel(const DiagArgs a)
 {
     __shared__ DiagShared<NJ, RT, GENERAL, ADJ> sh;
-    diag_tile<NJ, RT, GENERAL, ADJ>(a, blockIdx.x, sh);
+    diag_tile<NJ, RT, GENERAL, ADJ, EO>(a, blockIdx.x, sh);
 }
 
 template <int NJ, int RT>
@@ -21,12 +21,14 @@ int launch_t(const DiagArgs &a, hipStream_t s)
     const int nfc = a.nx * a.M;
     const int blocks = (nfc + fc_per_tile - 1) / fc_per_tile;
     const bool general = a.ne > 1 || a.alpha_l != 0.0 || a.alpha_r != 0.0 || a.periodic;
+    if (a.adjoint ? a.eo : a.eo == general)
+        return set_error(HZ_ERR_ARG, "DIAG form: the even/odd basis serves forward one-element Dirichlet solves");
     if (a.adjoint) {
-        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true, false>), dim3(blocks), dim3(256), 0, s, a);
     } else if (general) {
-        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true, false>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true, false, false>), dim3(blocks), dim3(256), 0, s, a);
     } else {
-        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, false>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, false, true>), dim3(blocks), dim3(256), 0, s, a);
     }
     HZ_HIP(hipGetLastError());
     return HZ_OK;

@@ -54,11 +54,11 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // latency overlaps the MFMAs of the steps in between.  ROLLED keeps the k
 // loop rolled (kernels that run several contractions in a loop would
 // otherwise spill the hoisted B loads).
-template <int NP, int LD, int RBK, int JT>
+template <int NP, int LD, int RBK, int JT, int KN = NP>
 __device__ __forceinline__ void tile_gemm_steps(d4 (&acc)[RBK][JT], const double *tile, const double *bcol, int ks,
                                                 double (&b0)[JT], double (&b1)[JT], int li, int lk)
 {
-    constexpr int NK = NP / 4;
+    constexpr int NK = KN / 4;
     double n0[JT], n1[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) n0[jt] = n1[jt] = 0.0;
@@ -91,11 +91,13 @@ __device__ __forceinline__ void tile_gemm_steps(d4 (&acc)[RBK][JT], const double
     }
 }
 
-template <int NP, int LD, int RBK, int JT, bool ROLLED = false>
+// KN: the contraction length (the k range of A's columns and B's rows that
+// the caller's pointers start at); B's row stride stays NP.
+template <int NP, int LD, int RBK, int JT, bool ROLLED = false, int KN = NP>
 __device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile, const double *__restrict__ B,
                                           int c0, int r0, int li, int lk)
 {
-    constexpr int NK = NP / 4;
+    constexpr int NK = KN / 4;
     static_assert(NK % 2 == 0, "two k-steps per iteration");
     const double *bcol = B + 16 * c0 + li + (long long)lk * NP;
     tile += 16 * r0 * LD;
@@ -107,9 +109,51 @@ __device__ __forceinline__ void tile_gemm(d4 (&acc)[RBK][JT], const double *tile
     }
     if constexpr (ROLLED) {
 #pragma nounroll
-        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT>(acc, tile, bcol, ks, b0, b1, li, lk);
+        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT, KN>(acc, tile, bcol, ks, b0, b1, li, lk);
     } else {
-        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT>(acc, tile, bcol, ks, b0, b1, li, lk);
+        for (int ks = 0; ks < NK; ks += 2) tile_gemm_steps<NP, LD, RBK, JT, KN>(acc, tile, bcol, ks, b0, b1, li, lk);
+    }
+}
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+// bf16 bits of x, rounded to nearest even through FP32.
+__device__ __forceinline__ short bf16_bits(double x)
+{
+    unsigned u = __float_as_uint((float)x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+// acc[rb][jt] += A B with v_mfma_f32_16x16x16bf16_1k: A = the FP64 LDS tile
+// rounded to bf16, B in bf16 from L2 laid out as Bq[k / 4][j][k % 4] (each
+// lane's four k values in 8 contiguous bytes), k < KN.  Operand layout (a
+// wave, 16 x 16 x 16): A[i][k] is element k % 4 of lane i + 16 (k / 4),
+// B[k][j] element k % 4 of lane j + 16 (k / 4); the result D[i][j] is
+// element i % 4 of lane j + 16 (i / 4), while the FP64 accumulators hold
+// D[i][j] in element i / 4 of lane j + 16 (i % 4): A's rows are fed
+// permuted, lane row i taking tile row 4 (i % 4) + i / 4, so that element r
+// of lane j + 16 lk is row 4 r + lk, the FP64 layout.
+template <int NP, int LD, int RBK, int JT, int KN>
+__device__ __forceinline__ void tile_gemm_bf16_rows(f4 (&acc)[RBK][JT], const double *tile, const s4 *__restrict__ Bq,
+                                                    int c0, int r0, int li, int lk)
+{
+    const s4 *bq = Bq + (long long)lk * NP + 16 * c0 + li;
+    const double *arow = tile + (16 * r0 + 4 * (li % 4) + li / 4) * LD + 4 * lk;
+#pragma unroll
+    for (int ks = 0; ks < KN / 16; ++ks) {
+        s4 bv[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) bv[jt] = bq[(long long)4 * ks * NP + 16 * jt];
+#pragma unroll
+        for (int rb = 0; rb < RBK; ++rb) {
+            const double *ap = arow + 16 * rb * LD + 16 * ks;
+            const s4 av = {bf16_bits(ap[0]), bf16_bits(ap[1]), bf16_bits(ap[2]), bf16_bits(ap[3])};
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[rb][jt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[jt], acc[rb][jt], 0, 0, 0);
+        }
     }
 }
 
@@ -125,12 +169,17 @@ struct DiagShared {
 };
 
 // Tile `bx` (RT rows) of the diagonalised mode solve; 256 threads.  Starts
-// with a barrier so that back-to-back tiles can reuse the LDS.
-template <int NJ, int RT, bool GENERAL, bool ADJ>
+// with a barrier so that back-to-back tiles can reuse the LDS.  EO: the
+// even/odd eigenbasis (a.eo; forward solves of one element with Dirichlet
+// faces), whose contractions run over half the k range.
+template <int NJ, int RT, bool GENERAL, bool ADJ, bool EO = !GENERAL && !ADJ>
 __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<NJ, RT, GENERAL, ADJ> &sh)
 {
+    static_assert(!(EO && (GENERAL || ADJ)), "the even/odd basis serves forward one-element Dirichlet solves");
     constexpr int NP = 16 * NJ;      // padded interior size
     constexpr int LD = NP + 2;       // LDS row stride (doubles)
+    constexpr int H = NP / 2;        // EO: the odd coordinates' offset
+    constexpr int KN = EO ? H : NP;  // contraction length
     // The four waves split the output's NJ column tiles WC ways and its RBK
     // row blocks WR ways; each wave computes JT column tiles of RBW row blocks.
     constexpr int WC = NJ % 4 == 0 ? 4 : 2, WR = 4 / WC;
@@ -154,10 +203,14 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     const int tid = threadIdx.x, lane = tid % WAVE, w = uniform(tid / WAVE);
     const int c0 = (w % WC) * JT, r0 = (w / WC) * RBW;  // the wave's first column tile, row block
     const int n = a.n, N = a.N, E = a.ne, K = a.K, M = a.M;
+    const int ho = n / 2, he = n - ho;                // EO: even and odd coordinates
     const int rows_per_fc = 2 * E;                    // re/im x elements
     const int fc_per_tile = RT / rows_per_fc;
     const int fc0 = bx * fc_per_tile;
     const int nfc = a.nx * M;
+    // EO: a wave's column tiles lie in one half, so its contractions run
+    // over that half's k range only (the eigenbasis is block diagonal)
+    const int kb = EO && 16 * c0 >= H ? H : 0;
 
     // ---- row bookkeeping: row = (fc_local*2 + plane)*E + e ----
     for (int row = tid; row < RT; row += 256) {
@@ -181,12 +234,15 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     }
     __syncthreads();
     // ---- stage F^T: tile[row][k] = f interior of the row's element ----
-    // Unrolled in chunks of up to 16 independent loads, all issued before
-    // any LDS store (a loop with one load per trip waits out the memory
-    // latency on every trip).
+    // EO: the folded interior, even coordinate i (k = i < H) f_i + f_{n-1-i}
+    // (f_i at the centre of an odd n), odd coordinate i (k = H + i)
+    // f_i - f_{n-1-i}.  Unrolled in chunks, all loads issued before any LDS
+    // store (a loop with one load per trip waits out the memory latency on
+    // every trip).
     {
-        constexpr int NIT = RT * NP / 256;  // loads per thread
-        constexpr int CH = NIT % 16 == 0 ? 16 : NIT % 12 == 0 ? 12 : NIT % 8 == 0 ? 8 : NIT;
+        constexpr int NIT = RT * NP / 256;  // tile entries per thread
+        constexpr int CH = EO ? (NIT % 8 == 0 ? 8 : NIT)
+                              : (NIT % 16 == 0 ? 16 : NIT % 12 == 0 ? 12 : NIT % 8 == 0 ? 8 : NIT);
         static_assert(NIT % CH == 0 && RT * NP % 256 == 0, "whole chunks");
 #pragma unroll
         for (int c0 = 0; c0 < NIT; c0 += CH) {
@@ -196,7 +252,18 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                 const int idx = tid + 256 * (c0 + it);
                 const int row = idx / NP, k = idx % NP;
                 const long long col = row_col[row];
-                v[it] = (col >= 0 && k < n) ? a.spec[col + 1 + k] : 0.0;
+                if constexpr (EO) {
+                    const bool even = k < H;
+                    const int i = even ? k : k - H;
+                    double x0 = 0.0, x1 = 0.0;
+                    if (col >= 0 && i < (even ? he : ho)) {
+                        x0 = a.spec[col + 1 + i];
+                        if (i < ho) x1 = a.spec[col + n - i];
+                    }
+                    v[it] = even ? x0 + x1 : x0 - x1;
+                } else {
+                    v[it] = (col >= 0 && k < n) ? a.spec[col + 1 + k] : 0.0;
+                }
             }
 #pragma unroll
             for (int it = 0; it < CH; ++it) {
@@ -214,7 +281,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
     const int li = lane % 16, lk = lane / 16;
-    tile_gemm<NP, LD, RBW, JT>(acc, tile, a.VinvT, c0, r0, li, lk);
+    tile_gemm<NP, LD, RBW, JT, false, KN>(acc, tile + kb, a.VinvT + (long long)kb * NP, c0, r0, li, lk);
     __syncthreads();  // every wave has read the F tile
 
     // ---- shift, interface sweep, assemble Z^T in the tile ----
@@ -231,14 +298,15 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (c0 + jt) + li;
-            const double mu = j < n ? a.mu[j] : 0.0;
-            const double v0 = (GENERAL || ADJ) && j < n ? dv0[j] : 0.0;
-            const double vN = (GENERAL || ADJ) && j < n ? dvN[j] : 0.0;
+            const bool jv = EO ? (j < H ? j < he : j - H < ho) : j < n;
+            const double mu = jv ? a.mu[j] : 0.0;
+            const double v0 = (GENERAL || ADJ) && jv ? dv0[j] : 0.0;
+            const double vN = (GENERAL || ADJ) && jv ? dvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * (r0 + rb) + 4 * r + lk;
                 const double d = mu - row_sig[row];
-                const double iv = (j < n) ? 1.0 / d : 0.0;
+                const double iv = jv ? 1.0 / d : 0.0;
                 inv[rb][jt][r] = iv;
                 if (GENERAL || ADJ) {
                     const double q = acc[rb][jt][r] * iv;
@@ -380,7 +448,8 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (c0 + jt) + li;
-            const double w0 = j < n ? zv0[j] : 0.0, wN = j < n ? zvN[j] : 0.0;
+            const bool jv = EO ? (j < H ? j < he : j - H < ho) : j < n;
+            const double w0 = jv ? zv0[j] : 0.0, wN = jv ? zvN[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * (r0 + rb) + 4 * r + lk;
@@ -389,25 +458,75 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             }
         }
     __syncthreads();
+    if constexpr (EO) {
+        // first-order correction from the symmetrised operator to the
+        // reference's: z -= (M z) / (mu - sigma), M z from the other half's
+        // coordinates (M maps even to odd and back), bf16 operands with FP32
+        // sums (M z is about 1e-12 |z|)
+        f4 cacc[RBW][JT];
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) cacc[rb][jt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        const int ko = H - kb;
+        tile_gemm_bf16_rows<NP, LD, RBW, JT, H>(cacc, tile + ko, reinterpret_cast<const s4 *>(a.eoM) + ko / 4 * NP,
+                                                c0, r0, li, lk);
+        __syncthreads();  // every wave has read the Z tile
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                const int j = 16 * (c0 + jt) + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int idx = (16 * (r0 + rb) + 4 * r + lk) * LD + j;
+                    tile[idx] = fma(-(double)cacc[rb][jt][r], inv[rb][jt][r], tile[idx]);
+                }
+            }
+        __syncthreads();
+    }
 
-    // ---- U^T = Z^T V^T, stored along z ----
+    // ---- U^T = Z^T V^T, stored along z (EO: unfolded first) ----
 #pragma unroll
     for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = d4{0.0, 0.0, 0.0, 0.0};
-    tile_gemm<NP, LD, RBW, JT>(acc, tile, a.VT, c0, r0, li, lk);
+    tile_gemm<NP, LD, RBW, JT, false, KN>(acc, tile + kb, a.VT + (long long)kb * NP, c0, r0, li, lk);
+    if constexpr (EO) {
+        __syncthreads();  // every wave has read the Z tile
 #pragma unroll
-    for (int rb = 0; rb < RBW; ++rb)
+        for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
-        for (int jt = 0; jt < JT; ++jt) {
-            const int k = 16 * (c0 + jt) + li;
+            for (int jt = 0; jt < JT; ++jt) {
+                const int k = 16 * (c0 + jt) + li;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * (r0 + rb) + 4 * r + lk;
-                const long long col = row_col[row];
-                if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
+                for (int r = 0; r < 4; ++r) tile[(16 * (r0 + rb) + 4 * r + lk) * LD + k] = acc[rb][jt][r];
             }
+        __syncthreads();
+        // u_i = e_i + o_i, u_{n-1-i} = e_i - o_i (i < n/2), the centre of an odd n e_i
+        for (int idx = tid; idx < RT * NP; idx += 256) {
+            const int row = idx / NP, k = idx % NP;
+            const long long col = row_col[row];
+            if (col < 0 || k >= n) continue;
+            const int i = k < n - 1 - k ? k : n - 1 - k;
+            const double e = tile[row * LD + i];
+            const double o = 2 * k + 1 == n ? 0.0 : tile[row * LD + H + i];
+            a.spec[col + 1 + k] = k <= n - 1 - k ? e + o : e - o;
         }
+    } else {
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                const int k = 16 * (c0 + jt) + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * (r0 + rb) + 4 * r + lk;
+                    const long long col = row_col[row];
+                    if (col >= 0 && k < n) a.spec[col + 1 + k] = acc[rb][jt][r];
+                }
+            }
+    }
     // element end values (the adjoint load has none: f is not used there)
     for (int row = tid; row < RT; row += 256) {
         const long long col = row_col[row];

@@ -207,4 +207,82 @@ int eigen_decompose(const std::vector<double> &A, int n, std::vector<double> &mu
     return HZ_OK;
 }
 
+// Even/odd split.  The GLL nodes are symmetric about the element centre, so
+// the interior operator A commutes with the reversal (J x)_i = x_{n-1-i} up
+// to the rounding of the nodes (|A - JAJ| is about 7e-13 |A| at n = 127).
+// The fold F x = (x_i + x_{n-1-i} | x_i - x_{n-1-i}), i < n/2, with the
+// centre entry of an odd n kept in the even part, takes the symmetrised
+// As = (A + JAJ)/2 to block-diagonal form F As F^-1 = diag(Ae, Ao), Ae of
+// size he = n - n/2 and Ao of size ho = n/2, and each block is diagonalised
+// on its own: a contraction with the eigenbasis then costs he^2 + ho^2
+// multiply-adds instead of n^2.  Padded layout (NP even, NP/2 >= he): even
+// coordinate i at index i, odd coordinate i at index NP/2 + i; W = diag(Ve,
+// Vo) and Winv = diag(Ve^-1, Vo^-1) (NP x NP, row-major, zero outside the
+// blocks), mu the eigenvalues (zero in the padding), so that
+//   As = F^-1 W diag(mu) Winv F,  F^-1 y = ((y_e + y_o)/2 | (y_e - y_o)/2),
+// the centre entry of an odd n taken from y_e.  *asym = max|A - JAJ| / max|A|.
+int eigen_decompose_split(const std::vector<double> &A, int n, int NP, std::vector<double> &mu,
+                          std::vector<double> &W, std::vector<double> &Winv, double *asym)
+{
+    const int m = n / 2, he = n - m, ho = m, H = NP / 2;
+    if (NP % 2 || H < he) return HZ_ERR_ARG;
+    ld amax = 0, dmax = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            amax = std::max(amax, (ld)std::fabs(A[(size_t)i * n + j]));
+            dmax = std::max(dmax, (ld)std::fabs((ld)A[(size_t)i * n + j] -
+                                                (ld)A[(size_t)(n - 1 - i) * n + (n - 1 - j)]));
+        }
+    if (asym) *asym = amax > 0 ? (double)(dmax / amax) : 0.0;
+    // B = F As F^-1 column by column: F^-1 of a unit vector has one or two
+    // entries, As x costs O(n), the fold O(n).
+    auto as = [&](int r, int c) {
+        return ((ld)A[(size_t)r * n + c] + (ld)A[(size_t)(n - 1 - r) * n + (n - 1 - c)]) / 2;
+    };
+    std::vector<ld> B((size_t)NP * NP, 0), y(n);
+    for (int c = 0; c < NP; ++c) {
+        const bool even = c < H;
+        const int j = even ? c : c - H;
+        if (j >= (even ? he : ho)) continue;
+        for (int r = 0; r < n; ++r) {
+            if (even && 2 * j + 1 == n) {
+                y[r] = as(r, j);
+            } else {
+                const ld s = even ? 1 : -1;
+                y[r] = (as(r, j) + s * as(r, n - 1 - j)) / 2;
+            }
+        }
+        for (int i = 0; i < he; ++i) B[(size_t)i * NP + c] = 2 * i + 1 == n ? y[i] : y[i] + y[n - 1 - i];
+        for (int i = 0; i < ho; ++i) B[(size_t)(H + i) * NP + c] = y[i] - y[n - 1 - i];
+    }
+    ld bmax = 0, off = 0;
+    for (int r = 0; r < NP; ++r)
+        for (int c = 0; c < NP; ++c) {
+            const ld v = std::fabs(B[(size_t)r * NP + c]);
+            bmax = std::max(bmax, v);
+            if ((r < H) != (c < H)) off = std::max(off, v);
+        }
+    if (off > bmax * 1e-12L) return HZ_ERR_UNSUPPORTED;  // not centrosymmetric
+    mu.assign(NP, 0.0);
+    W.assign((size_t)NP * NP, 0.0);
+    Winv.assign((size_t)NP * NP, 0.0);
+    for (int half = 0; half < 2; ++half) {
+        const int b = half ? ho : he, o = half ? H : 0;
+        if (!b) continue;
+        std::vector<double> blk((size_t)b * b), ev, V, Vi;
+        for (int i = 0; i < b; ++i)
+            for (int j = 0; j < b; ++j) blk[(size_t)i * b + j] = (double)B[(size_t)(o + i) * NP + (o + j)];
+        const int st = eigen_decompose(blk, b, ev, V, Vi);
+        if (st) return st;
+        for (int i = 0; i < b; ++i) {
+            mu[o + i] = ev[i];
+            for (int j = 0; j < b; ++j) {
+                W[(size_t)(o + i) * NP + (o + j)] = V[(size_t)i * b + j];
+                Winv[(size_t)(o + i) * NP + (o + j)] = Vi[(size_t)i * b + j];
+            }
+        }
+    }
+    return HZ_OK;
+}
+
 }  // namespace hz

@@ -22,6 +22,8 @@ int plan_operators(int N, int ne, double Lz, std::vector<double> &z, std::vector
 double mode_shift(double lambda, int iw, int jw, double Lx, double Ly);
 int eigen_decompose(const std::vector<double> &A, int n, std::vector<double> &mu,
                     std::vector<double> &V, std::vector<double> &Vinv);
+int eigen_decompose_split(const std::vector<double> &A, int n, int NP, std::vector<double> &mu,
+                          std::vector<double> &W, std::vector<double> &Winv, double *asym);
 
 // ---- error reporting --------------------------------------------------------
 int set_error(int code, const std::string &msg);
@@ -95,11 +97,14 @@ struct DiagArgs {
     long long plane_off;
     const double *bc_re, *bc_im;
     const double *shift;        // [modes]
-    const double *mu, *w0, *wN, *r0, *rN;  // [n]
+    const double *mu, *w0, *wN, *r0, *rN;  // [n], or [NP] with eo
     const double *VinvT, *VT;   // [NP][NP], NP = diag_padded(n): n rounded up to 32 (256 above 128)
     const double *hom;          // [modes][4]: Dz0.ul, DzN.ul, Dz0.ur, DzN.ur
     double alpha_l, beta_l, alpha_r, beta_r;
     int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
+    int eo;                     // even/odd eigenbasis (forward, one element, Dirichlet faces):
+                                // the vectors padded to NP, the operands block diagonal
+    const short *eoM;           // eo: the correction to the reference's operator, bf16 [NP/4][NP][4]
     double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
     int rt;                     // tile rows (spectral columns x planes x elements); 0 = default
     int periodic;               // z periodic: cyclic interface system, no face data
@@ -190,6 +195,8 @@ FftArgs xy_pass_args(const hz_plan *p, int pass, const double *in, double *re, d
 inline long long spectrum_xs(int M, int K) { return ((long long)M * K + 15) / 16 * 16; }
 // Arguments of the forward diagonalised mode solve on `p`'s workspace.
 DiagArgs diag_args(const hz_plan *p);
+bool diag_general(const hz_plan *p);  // several elements, Robin faces or periodic z
+void diag_basis(const hz_plan *p, DiagArgs &a, bool adjoint);
 // Arguments of the MIXED mode solve on `p`'s workspace.
 MixedArgs mixed_args(const hz_plan *p);
 }  // namespace hz
@@ -214,10 +221,17 @@ struct hz_plan {
     double *d_closure = nullptr;  // periodic z: closure coefficient per real spectral column
     // z-diagonalised form
     int NP = 0;
+    // full eigenbasis (n entries, V and Vinv n x n; the device operands
+    // padded to NP x NP)
     std::vector<double> mu, V, Vinv, w0, wN, r0, rN;
     double *d_mu = nullptr, *d_w0 = nullptr, *d_wN = nullptr, *d_r0 = nullptr, *d_rN = nullptr;
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
+    // even/odd eigenbasis (forward solves of one element with Dirichlet
+    // faces): mu, w0, wN, r0, rN padded to NP each; operands NP x NP
+    std::vector<double> eo_vec;
+    double *d_eo_vec = nullptr, *d_eo_B1 = nullptr, *d_eo_B2 = nullptr, *d_eo_hom = nullptr;
+    short *d_eo_M = nullptr;  // first-order correction to the reference's operator (bf16)
     int diag_rt = 0;                                  // autotuned tile rows
     int fft_w[4] = {0, 0, 0, 0};                      // autotuned lanes per pass
     double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass

@@ -44,8 +44,6 @@
 namespace hz {
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-
 // acc[rb][jt] += A B in FP32: A = the FP32 LDS tile (rows 16 (r0 + rb) + i,
 // column k, row stride LDF), B[k][j] = B[k * NP + j] from L2, loaded two
 // k-steps ahead (as tile_gemm); the k loop stays rolled (the solve runs it

@@ -77,8 +77,10 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
     for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
-                      p->d_Vinvrow, p->d_partial, p->d_DzzTP, p->d_UL, p->d_UR, p->d_hist, p->d_closure})
+                      p->d_Vinvrow, p->d_eo_vec, p->d_eo_B1, p->d_eo_B2, p->d_eo_hom, p->d_partial, p->d_DzzTP,
+                      p->d_UL, p->d_UR, p->d_hist, p->d_closure})
         (void)hipFree(q);
+    (void)hipFree(p->d_eo_M);
     (void)hipFree(p->d_VinvT32);
     (void)hipFree(p->d_VT32);
     (void)hipFree(p->d_xspec);
@@ -87,30 +89,43 @@ void free_plan(hz_plan *p)
     delete p;
 }
 
-// Per-mode end-point derivatives of the homogeneous solutions in the
-// diagonalised basis: Dz[0|N-1].ul and Dz[0|N-1].ur for the mode's shift.
-int compute_homogeneous(hz_plan *p)
+// Per-mode end-point derivatives of the homogeneous solutions for the mode's
+// shift: Dz[0|N-1].ul and Dz[0|N-1].ur, in the full eigenbasis and in the
+// even/odd one (the diag kernel's one-element Dirichlet solves).
+static void homogeneous(const hz_plan *p, int len, const double *mu, const double *w0, const double *wN,
+                        const double *r0, const double *rN, std::vector<double> &hom)
 {
-    if (!p->d_hom) return HZ_OK;
-    const int n = p->n, N = p->N;
-    std::vector<double> hom((size_t)4 * p->modes);
+    const int N = p->N;
+    hom.assign((size_t)4 * p->modes, 0.0);
     for (int m = 0; m < p->modes; ++m) {
         long double a0 = p->Dz[0], aN = p->Dz[(size_t)(N - 1) * N];
         long double b0 = p->Dz[N - 1], bN = p->Dz[(size_t)(N - 1) * N + N - 1];
         const long double sig = p->shift[m];
-        for (int j = 0; j < n; ++j) {
-            const long double iv = 1.0L / ((long double)p->mu[j] - sig);
-            a0 += (long double)p->r0[j] * p->w0[j] * iv;
-            aN += (long double)p->rN[j] * p->w0[j] * iv;
-            b0 += (long double)p->r0[j] * p->wN[j] * iv;
-            bN += (long double)p->rN[j] * p->wN[j] * iv;
+        for (int j = 0; j < len; ++j) {
+            if (w0[j] == 0 && wN[j] == 0) continue;  // padding (and loads with no component)
+            const long double iv = 1.0L / ((long double)mu[j] - sig);
+            a0 += (long double)r0[j] * w0[j] * iv;
+            aN += (long double)rN[j] * w0[j] * iv;
+            b0 += (long double)r0[j] * wN[j] * iv;
+            bN += (long double)rN[j] * wN[j] * iv;
         }
         hom[4 * m + 0] = (double)a0;
         hom[4 * m + 1] = (double)aN;
         hom[4 * m + 2] = (double)b0;
         hom[4 * m + 3] = (double)bN;
     }
+}
+
+int compute_homogeneous(hz_plan *p)
+{
+    if (!p->d_hom) return HZ_OK;
+    std::vector<double> hom;
+    homogeneous(p, p->n, p->mu.data(), p->w0.data(), p->wN.data(), p->r0.data(), p->rN.data(), hom);
     HZ_HIP(hipMemcpy(p->d_hom, hom.data(), sizeof(double) * hom.size(), hipMemcpyHostToDevice));
+    const double *v = p->eo_vec.data();
+    const int NP = p->NP;
+    homogeneous(p, NP, v, v + NP, v + 2 * NP, v + 3 * NP, v + 4 * NP, hom);
+    HZ_HIP(hipMemcpy(p->d_eo_hom, hom.data(), sizeof(double) * hom.size(), hipMemcpyHostToDevice));
     return HZ_OK;
 }
 
@@ -176,6 +191,158 @@ int compute_shifts(hz_plan *p)
     return HZ_OK;
 }
 
+// The diag kernel's operands in the even/odd eigenbasis (eigen_decompose_split)
+// for one element with Dirichlet faces.  With F the fold, F^-1 its inverse
+// and W, Winv, mu the decomposition of As = (A + JAJ)/2 (padded to NP),
+//   z = (Winv F f + zl w0 + zr wN) / (mu - sigma),  w0|wN = Winv F (-Dzz[1..N-2][0|N-1]),
+// and u = F^-1 W z solves with As.  The reference's A = As + Ea, Ea =
+// (A - JAJ)/2 of the size of the GLL nodes' rounding, so to first order
+//   (A - sigma)^-1 = F^-1 W (I - Lambda^-1 M) Lambda^-1 Winv F,  M = Winv F Ea F^-1 W,
+// Lambda = diag(mu - sigma): the kernel solves with A by z -= (M z) / (mu -
+// sigma) before the second contraction (M z is about 1e-12 |z|, so bf16
+// operands with FP32 sums suffice; without the correction the solution moves by up to 2e-12
+// relative at N = 257).  M maps even to odd coordinates and back (Ea
+// anticommutes with J).  The kernel applies F and F^-1 as sums and
+// differences of mirrored entries (the halves of F^-1 are folded into D W,
+// D = 1/2 except on the centre row of an odd n).  Operands, NP x NP
+// row-major: B1 = Winv^T, B2 = (D W)^T (FP64), M in bf16 (correction_operator); r0|rN =
+// Dz[0|N-1][1..N-2] F^-1 W (the homogeneous coefficients' basis).
+std::vector<short> correction_operator(const std::vector<double> &A, int n, int NP, const std::vector<double> &W,
+                                       const std::vector<double> &Wi);
+
+int setup_even_odd(hz_plan *p, const std::vector<double> &A)
+{
+    const int n = p->n, N = p->N, NP = p->NP, H = NP / 2, m = n / 2;
+    static std::mutex cache_mu;
+    static std::map<std::pair<std::vector<double>, int>, std::array<std::vector<double>, 3>> cache;
+    static std::map<std::pair<std::vector<double>, int>, std::vector<short>> cache_m;
+    std::vector<double> mu, W, Wi;
+    std::vector<short> Mq;
+    const auto key = std::make_pair(A, NP);
+    {
+        std::lock_guard<std::mutex> lk(cache_mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            mu = it->second[0];
+            W = it->second[1];
+            Wi = it->second[2];
+            Mq = cache_m[key];
+        } else {
+            const int st = eigen_decompose_split(A, n, NP, mu, W, Wi, nullptr);
+            if (st) return set_error(st, "even/odd eigendecomposition of the GLL operator failed");
+        }
+    }
+    // padded index of the fold's coordinates: even i -> i, odd i -> H + i
+    auto fold = [&](const std::vector<long double> &x) {
+        std::vector<long double> y(NP, 0);
+        for (int i = 0; i < m; ++i) {
+            y[i] = x[i] + x[n - 1 - i];
+            y[H + i] = x[i] - x[n - 1 - i];
+        }
+        if (n % 2) y[m] = x[m];
+        return y;
+    };
+    auto unfold_row = [&](const std::vector<long double> &d) {  // d F^-1
+        std::vector<long double> y(NP, 0);
+        for (int i = 0; i < m; ++i) {
+            y[i] = (d[i] + d[n - 1 - i]) / 2;
+            y[H + i] = (d[i] - d[n - 1 - i]) / 2;
+        }
+        if (n % 2) y[m] = d[m];
+        return y;
+    };
+    std::vector<long double> g0(n), gN(n), d0(n), dN(n);
+    for (int k = 0; k < n; ++k) {
+        g0[k] = -p->Dzz[(size_t)(k + 1) * N];
+        gN[k] = -p->Dzz[(size_t)(k + 1) * N + N - 1];
+        d0[k] = p->Dz[k + 1];
+        dN[k] = p->Dz[(size_t)(N - 1) * N + k + 1];
+    }
+    const std::vector<long double> fg0 = fold(g0), fgN = fold(gN), ud0 = unfold_row(d0), udN = unfold_row(dN);
+    p->eo_vec.assign((size_t)5 * NP, 0.0);
+    double *v = p->eo_vec.data();
+    for (int j = 0; j < NP; ++j) {
+        long double s0 = 0, sN = 0, t0 = 0, tN = 0;
+        for (int c = 0; c < NP; ++c) {
+            s0 += (long double)Wi[(size_t)j * NP + c] * fg0[c];
+            sN += (long double)Wi[(size_t)j * NP + c] * fgN[c];
+            t0 += ud0[c] * W[(size_t)c * NP + j];
+            tN += udN[c] * W[(size_t)c * NP + j];
+        }
+        v[j] = mu[j];
+        v[NP + j] = (double)s0;
+        v[2 * NP + j] = (double)sN;
+        v[3 * NP + j] = (double)t0;
+        v[4 * NP + j] = (double)tN;
+    }
+    std::vector<double> B1((size_t)NP * NP), B2((size_t)NP * NP);
+    for (int r = 0; r < NP; ++r) {
+        const double dr = (n % 2 && r == m) ? 1.0 : 0.5;
+        for (int c = 0; c < NP; ++c) {
+            B1[(size_t)c * NP + r] = Wi[(size_t)r * NP + c];
+            B2[(size_t)c * NP + r] = dr * W[(size_t)r * NP + c];  // (D W)[r][c]
+        }
+    }
+    if (Mq.empty()) {
+        Mq = correction_operator(A, n, NP, W, Wi);
+        std::lock_guard<std::mutex> lk(cache_mu);
+        if (cache.size() < 256) {
+            cache[key] = {mu, W, Wi};
+            cache_m[key] = Mq;
+        }
+    }
+    int st;
+    if ((st = upload(&p->d_eo_vec, p->eo_vec)) || (st = upload(&p->d_eo_B1, B1)) || (st = upload(&p->d_eo_B2, B2)) ||
+        (st = upload(&p->d_eo_M, Mq)))
+        return st;
+    return HZ_OK;
+}
+
+// The even/odd solve's first-order correction M = Winv (F Ea F^-1) W, Ea =
+// (A - JAJ)/2 (the middle factor column by column: F^-1 of a unit vector has
+// one or two entries; then the two products, in long double), as the bf16
+// B operand of c = M z: Mq[(k / 4) * NP + j][k % 4] = bf16(M[j][k]),
+// rounded to nearest even.
+std::vector<short> correction_operator(const std::vector<double> &A, int n, int NP, const std::vector<double> &W,
+                                       const std::vector<double> &Wi)
+{
+    const int H = NP / 2, m = n / 2;
+    typedef long double ld;
+    std::vector<ld> Et((size_t)NP * NP, 0), T((size_t)NP * NP, 0);
+    std::vector<ld> y(n);
+    auto ea = [&](int r, int c) { return ((ld)A[(size_t)r * n + c] - (ld)A[(size_t)(n - 1 - r) * n + (n - 1 - c)]) / 2; };
+    for (int c = 0; c < NP; ++c) {
+        const bool even = c < H;
+        const int j = even ? c : c - H;
+        if (j >= (even ? n - m : m)) continue;
+        for (int r = 0; r < n; ++r)
+            y[r] = even && 2 * j + 1 == n ? ea(r, j) : (ea(r, j) + (even ? 1 : -1) * ea(r, n - 1 - j)) / 2;
+        for (int i = 0; i < m; ++i) {
+            Et[(size_t)i * NP + c] = y[i] + y[n - 1 - i];
+            Et[(size_t)(H + i) * NP + c] = y[i] - y[n - 1 - i];
+        }
+        if (n % 2) Et[(size_t)m * NP + c] = y[m];
+    }
+    for (int r = 0; r < NP; ++r)      // T = Et W
+        for (int k = 0; k < NP; ++k) {
+            const ld e = Et[(size_t)r * NP + k];
+            if (e == 0) continue;
+            for (int c = 0; c < NP; ++c) T[(size_t)r * NP + c] += e * (ld)W[(size_t)k * NP + c];
+        }
+    std::vector<short> Mq((size_t)NP * NP, 0);
+    for (int r = 0; r < NP; ++r)      // M = Winv T
+        for (int c = 0; c < NP; ++c) {
+            ld acc = 0;
+            for (int k = 0; k < NP; ++k) acc += (ld)Wi[(size_t)r * NP + k] * T[(size_t)k * NP + c];
+            const float f = (float)acc;
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            u += 0x7FFFu + ((u >> 16) & 1u);
+            Mq[((size_t)(c / 4) * NP + r) * 4 + c % 4] = (short)(u >> 16);
+        }
+    return Mq;
+}
+
 // Eigen-decomposition of the interior operator and the derived vectors of
 // the diagonalised form, uploaded padded to NP x NP.
 int setup_diag(hz_plan *p)
@@ -231,10 +398,11 @@ int setup_diag(hz_plan *p)
             Vrow[(size_t)i * NP + j] = p->V[(size_t)i * n + j];
             Vinvrow[(size_t)i * NP + j] = p->Vinv[(size_t)i * n + j];
         }
+    if ((st = setup_even_odd(p, A))) return st;
     if ((st = upload(&p->d_mu, p->mu)) || (st = upload(&p->d_w0, p->w0)) || (st = upload(&p->d_wN, p->wN)) ||
         (st = upload(&p->d_r0, p->r0)) || (st = upload(&p->d_rN, p->rN)) || (st = upload(&p->d_VinvT, VinvT)) ||
         (st = upload(&p->d_VT, VT)) || (st = upload(&p->d_Vrow, Vrow)) || (st = upload(&p->d_Vinvrow, Vinvrow)) ||
-        (st = dmalloc(&p->d_hom, (size_t)4 * p->modes)))
+        (st = dmalloc(&p->d_hom, (size_t)4 * p->modes)) || (st = dmalloc(&p->d_eo_hom, (size_t)4 * p->modes)))
         return st;
     return HZ_OK;
 }
@@ -388,6 +556,40 @@ int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipSt
 }  // namespace
 
 // Kernel arguments of the z-diagonalised mode solve (forward map).
+// The diag kernel's operands: the even/odd eigenbasis for the forward solve
+// of one element with Dirichlet faces (the kernels' EO instances), the full
+// eigenbasis otherwise.
+void hz::diag_basis(const hz_plan *p, DiagArgs &a, bool adjoint)
+{
+    const int NP = p->NP;
+    a.eo = !adjoint && !diag_general(p);
+    a.eoM = p->d_eo_M;
+    if (a.eo) {
+        a.mu = p->d_eo_vec;
+        a.w0 = p->d_eo_vec + NP;
+        a.wN = p->d_eo_vec + 2 * NP;
+        a.r0 = p->d_eo_vec + 3 * NP;
+        a.rN = p->d_eo_vec + 4 * NP;
+        a.VinvT = p->d_eo_B1;
+        a.VT = p->d_eo_B2;
+        a.hom = p->d_eo_hom;
+    } else {
+        a.mu = p->d_mu;
+        a.w0 = p->d_w0;
+        a.wN = p->d_wN;
+        a.r0 = p->d_r0;
+        a.rN = p->d_rN;
+        a.VinvT = adjoint ? p->d_Vrow : p->d_VinvT;
+        a.VT = adjoint ? p->d_Vinvrow : p->d_VT;
+        a.hom = p->d_hom;
+    }
+}
+
+bool hz::diag_general(const hz_plan *p)
+{
+    return p->ne > 1 || p->d.alpha_l != 0.0 || p->d.alpha_r != 0.0 || p->d.periodic_z;
+}
+
 DiagArgs hz::diag_args(const hz_plan *p)
 {
     DiagArgs a{};
@@ -403,14 +605,7 @@ DiagArgs hz::diag_args(const hz_plan *p)
     a.bc_re = p->bc.re;
     a.bc_im = p->bc.im;
     a.shift = p->d_shift;
-    a.mu = p->d_mu;
-    a.w0 = p->d_w0;
-    a.wN = p->d_wN;
-    a.r0 = p->d_r0;
-    a.rN = p->d_rN;
-    a.VinvT = p->d_VinvT;
-    a.VT = p->d_VT;
-    a.hom = p->d_hom;
+    diag_basis(p, a, false);
     a.alpha_l = p->d.alpha_l;
     a.beta_l = p->d.beta_l;
     a.alpha_r = p->d.alpha_r;
@@ -837,8 +1032,7 @@ static int adjoint_pipeline(hz_plan *p, const double *ubar, double *fbar, double
     if (rec) rec->mark("fft_forward");
     DiagArgs a = diag_args(p);
     a.adjoint = 1;
-    a.VinvT = p->d_Vrow;
-    a.VT = p->d_Vinvrow;
+    diag_basis(p, a, true);
     a.gbar_re = p->bc.re;
     a.gbar_im = p->bc.im;
     if ((st = launch_diag_modes(a, s))) return st;
@@ -1063,6 +1257,27 @@ int hz_host_eigen(int n, int ne, double lz, double *mu, double *V, double *Vinv)
     if (mu) std::memcpy(mu, ev.data(), sizeof(double) * m);
     if (V) std::memcpy(V, vv.data(), sizeof(double) * vv.size());
     if (Vinv) std::memcpy(Vinv, vi.data(), sizeof(double) * vi.size());
+    return HZ_OK;
+}
+
+int hz_host_eigen_split(int n, int ne, double lz, int np, double *mu, double *W, double *Winv, double *asym)
+{
+    if (n < 3) return set_error(HZ_ERR_N_TOO_SMALL, "need n >= 3");
+    const int m = n - 2;
+    if (np % 2 || np / 2 < m - m / 2) return set_error(HZ_ERR_ARG, "np must be even with np/2 >= ceil((n-2)/2)");
+    std::vector<double> zz, a, b;
+    int st = plan_operators(n, ne, lz, zz, a, b);
+    if (st) return set_error(st, "GLL setup failed");
+    std::vector<double> A((size_t)m * m), ev, w, wi;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = b[(size_t)(i + 1) * n + (j + 1)];
+    double as = 0;
+    st = eigen_decompose_split(A, m, np, ev, w, wi, &as);
+    if (st) return set_error(st, "even/odd eigendecomposition failed");
+    if (mu) std::memcpy(mu, ev.data(), sizeof(double) * ev.size());
+    if (W) std::memcpy(W, w.data(), sizeof(double) * w.size());
+    if (Winv) std::memcpy(Winv, wi.data(), sizeof(double) * wi.size());
+    if (asym) *asym = as;
     return HZ_OK;
 }
 

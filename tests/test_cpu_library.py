@@ -78,6 +78,57 @@ def test_host_eigendecomposition(N):
 
 
 
+@pytest.mark.parametrize("N,NP", [(9, 32), (10, 32), (33, 32), (34, 64), (129, 128), (257, 256)])
+def test_host_even_odd_eigendecomposition(N, NP):
+    """The DIAG kernel's even/odd eigenbasis: F^-1 W diag(mu) Winv F is the
+    symmetrised interior operator (A + JAJ)/2 to round-off, which is A up to
+    the rounding of the GLL nodes; W and Winv are block diagonal and inverse
+    to each other; the eigenvalues are those of A.  And the solve it gives,
+    F^-1 W (mu - s)^-1 Winv F f, is numpy's LU solve of A - s to 1e-13
+    (relative to max|u|) for the shifts a 256^2 plan spans."""
+    from mpi_amd import _lib
+    lib = _lib.load()
+    P = ctypes.POINTER(ctypes.c_double)
+    m = N - 2
+    mu = np.zeros(NP)
+    W = np.zeros((NP, NP))
+    Wi = np.zeros((NP, NP))
+    asym = ctypes.c_double()
+    _lib.check(lib.hz_host_eigen_split(N, 1, 2.0 * math.pi, NP, mu.ctypes.data_as(P), W.ctypes.data_as(P),
+                                       Wi.ctypes.data_as(P), ctypes.byref(asym)))
+    Dzz = np.zeros((N, N))
+    _lib.check(lib.hz_host_plan_operators(N, 1, 2.0 * math.pi, None, None, None, Dzz.ctypes.data_as(P)))
+    A = Dzz[1:-1, 1:-1]
+    J = np.eye(m)[::-1]
+    As = (A + J @ A @ J) / 2
+    assert asym.value == np.abs(A - J @ A @ J).max() / np.abs(A).max() and asym.value < 1e-11
+    H, h = NP // 2, m // 2
+    he = m - h
+    F = np.zeros((NP, m))
+    for i in range(h):
+        F[i, i] = F[i, m - 1 - i] = 1.0
+        F[H + i, i], F[H + i, m - 1 - i] = 1.0, -1.0
+    if m % 2:
+        F[h, h] = 1.0
+    Fi = np.linalg.pinv(F)  # exact left inverse: halves of sums and differences
+    valid = np.zeros(NP, bool)
+    valid[:he] = True
+    valid[H:H + h] = True
+    assert not mu[~valid].any() and not W[~valid].any() and not W[:, ~valid].any() and not Wi[~valid].any()
+    assert not W[:H, H:].any() and not W[H:, :H].any() and not Wi[:H, H:].any() and not Wi[H:, :H].any()
+    assert np.abs(W @ Wi - np.diag(valid.astype(float))).max() <= 1e-13
+    R = Fi @ W @ np.diag(mu) @ Wi @ F
+    assert np.abs(R - As).max() <= 1e-13 * np.abs(A).max()
+    assert np.allclose(np.sort(mu[valid]), np.sort(np.linalg.eigvals(A).real), rtol=1e-10, atol=0)
+    z = np.linspace(0.0, 2.0 * math.pi, m + 2)[1:-1]
+    f = np.sin(3 * z) * np.exp(np.cos(z))
+    for s in (10.0, 12.0, 1e3, 10.0 + 2 * 128.0 ** 2):
+        inv = np.where(valid, 1.0 / np.where(valid, mu - s, 1.0), 0.0)
+        u = Fi @ (W @ (inv * (Wi @ (F @ f))))
+        ul = np.linalg.solve(A - s * np.eye(m), f)
+        assert np.abs(u - ul).max() <= 1e-13 * np.abs(ul).max() * max(1.0, m / 64)
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py --gpus 2 exits with status 2 and one JSON line naming the
     reason, before it touches a device."""
