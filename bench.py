@@ -137,8 +137,13 @@ def measured_traffic(stage):
     except (OSError, ValueError, KeyError):
         return None
     prefix = STAGE_KERNELS.get(stage, stage)
+
+    def adjoint_instance(name):  # diag_modes_kernel<NJ, RT, GENERAL, ADJ, EO>
+        args = [a.strip() for a in name[name.find("<") + 1:name.rfind(">")].split(",")]
+        return name.startswith("diag_modes_kernel") and len(args) > 3 and args[3] == "true"
+
     cands = [(v["dispatches"], v["hbm_bytes_per_dispatch"]) for k, v in kernels.items()
-             if k.startswith(prefix) and "true>" not in k and v.get("hbm_bytes_per_dispatch")]
+             if k.startswith(prefix) and not adjoint_instance(k) and v.get("hbm_bytes_per_dispatch")]
     return max(cands)[1] if cands else None
 
 
@@ -532,14 +537,22 @@ def main():
                 "unit": "TFLOP/s", "work_per_launch": work, "work_unit": "FP64 flop",
                 "note": "FP64 vector FMA path; FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
     elif stage.startswith("diag"):
+        # the kernel's own time: back-to-back launches timed by the plan's
+        # tile tuner (HIP events on its stream, 3 launches of the chosen tile);
+        # the stage's event pair inside a solve also holds the idle gap an
+        # event pair between two kernels costs
+        tun = chosen.get("tuning") or {}
+        tk = (tun.get("choices") or {}).get(str(tun.get("diag_tile_rows")))
+        stage_event_ms, ms = ms, (tk if tk else ms)
         work = diag_flops(NX, NY, N)
         dense = diag_flops(NX, NY, N, even_odd=False)
         roof = {"kernel": stage, "bound": "mfma", "achieved": work / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "work_per_launch": work, "work_unit": "FP64 flop",
-                "dense_equivalent_tflops": dense / (ms * 1e-3) / 1e12,
+                "dense_equivalent_tflops": dense / (ms * 1e-3) / 1e12, "stage_event_ms": stage_event_ms,
                 "note": "work = the even/odd split's two half-size contractions per column, the flop this kernel "
                         "executes; dense_equivalent_tflops prices the same time at the full eigenbasis's "
-                        "4n^2 + 8n per column"}
+                        "4n^2 + 8n per column; kernel_ms = back-to-back launches of the tuned tile (the plan's "
+                        "tuner, HIP events), stage_event_ms = the stage's event pair inside a solve"}
     else:
         work = fft_bytes(NX, NY, K) / 2
         roof = {"kernel": stage, "bound": "hbm", "achieved": work / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,

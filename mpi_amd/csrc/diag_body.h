@@ -25,6 +25,19 @@
 //   fbar_int = V^{-T} ((y + P0b r0 + PNb rN) * inv),  fbar = 0 at the ends.
 // The kernel is the forward one with B1 = V, B2 = V^{-1}.
 //
+// Even/odd instance (EO: forward solves of one element with Dirichlet
+// faces).  The GLL nodes mirror about the element centre, so Dzz_int
+// commutes with the reversal J up to the nodes' rounding; with the fold
+// F f = (f_i + f_{n-1-i} | f_i - f_{n-1-i}) the symmetrised operator
+// (A + JAJ)/2 is block diagonal and is diagonalised block by block
+// (eigen_decompose_split): V -> F^{-1} W, V^{-1} -> Winv F with W, Winv
+// block diagonal (even coordinates at 0.., odd ones at NP/2..).  Each wave's
+// output columns lie in one half, so its contractions run over that half's
+// k range: half the matrix work.  The tile is folded while it is staged and
+// unfolded through LDS before the store; between the contractions
+// z -= (M z) / (mu - sigma) (M = Winv F (A - JAJ)/2 F^{-1} W, bf16 operands,
+// FP32 sums) corrects to first order from the symmetrised operator to A.
+//
 // A work-group owns RT tile rows (spectral column x element) and all NP
 // (padded n) eigen/grid indices.  Contractions are transposed so that MFMA
 // output lanes run along z, which is contiguous in memory:
