@@ -390,8 +390,9 @@ def time_precision(steps=5):
         return ms, float(np.abs(got - want).max() / rec["max_abs_u"])
 
     ms, err = run(plan)
+    tuning = plan.tuning()
     plan.close()
-    res["diag"] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms, "rel_err_vs_reference": err}
+    res["diag"] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms, "rel_err_vs_reference": err, "tuning": tuning}
     per_sweep = []
     for sweeps in (-1, 1, 2, 3):
         p = HelmholtzPlan(512, 512, 257, 1, lam=100.0, form="mixed", refine_sweeps=sweeps)

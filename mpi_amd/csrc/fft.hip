@@ -122,6 +122,11 @@ int launch_unit(FftKind kind, const FftArgs &a, hipStream_t s)
 
 int fft_width_choices(int L, int *w, int max)
 {
+    if (L == 512) {  // 64 threads per line: 16 lanes fill a 1024-thread work-group
+        static const int c[] = {8, 16};
+        for (int i = 0; i < 2 && i < max; ++i) w[i] = c[i];
+        return 2;
+    }
     if (L < 64 || L > 256) return 0;
     static const int c[] = {8, 16, 32};
     for (int i = 0; i < 3 && i < max; ++i) w[i] = c[i];
@@ -149,7 +154,7 @@ int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s)
                : a.w == 32 ? launch_kind<256, 16, 16, 32>(kind, a, s) : launch_kind<256, 16, 16, 16>(kind, a, s);
     // three stages (fft_core3): 8 x 8 x 8 and 16 x 8 x 8, 59-66 and 110-114
     // VGPRs (the two-stage 32 x 16 and 32 x 32 splits held 206-256)
-    case 512: return launch_kind<512, 8, 64, 16, 8>(kind, a, s);
+    case 512: return a.w == 8 ? launch_kind<512, 8, 64, 8, 8>(kind, a, s) : launch_kind<512, 8, 64, 16, 8>(kind, a, s);
     case 1024: return launch_kind<1024, 16, 64, 8, 8>(kind, a, s);
     default: return set_error(HZ_ERR_UNSUPPORTED, "FFT length must be a power of two in [1, 1024]");
     }

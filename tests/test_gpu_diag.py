@@ -76,9 +76,13 @@ def test_diag_config2_multi_element(cuda):
     assert np.abs(got - want).max() <= TOL * rec["max_abs_u"]
 
 
-@pytest.mark.parametrize("N,ne,robin", [(17, 3, True), (65, 1, True), (257, 1, False), (130, 1, True)])
+@pytest.mark.parametrize("N,ne,robin", [(17, 3, True), (65, 1, True), (257, 1, False), (130, 1, True),
+                                        (130, 1, False), (34, 1, False), (6, 1, False), (5, 1, False)])
 def test_diag_general_against_oracle(cuda, N, ne, robin):
-    """Robin faces, several elements, and the largest element size (N = 257)."""
+    """Robin faces, several elements, and the largest element size (N = 257);
+    one Dirichlet element (the even/odd split with its first-order
+    correction) with an even interior size (no centre entry: N = 130, 34, 6)
+    and the smallest odd one (N = 5, one odd coordinate)."""
     from mpi_amd.solver import HelmholtzPlan
     nx, ny = 16, 8
     al, bl, ar, br = (0.7, 1.3, 0.4, 2.0) if robin else (0.0, 1.0, 0.0, 1.0)
