@@ -21,9 +21,10 @@ int launch_t(const DiagArgs &a, hipStream_t s)
     const int nfc = a.nx * a.M;
     const int blocks = (nfc + fc_per_tile - 1) / fc_per_tile;
     const bool general = a.ne > 1 || a.alpha_l != 0.0 || a.alpha_r != 0.0 || a.periodic;
-    if (a.adjoint ? a.eo : a.eo == general)
-        return set_error(HZ_ERR_ARG, "DIAG form: the even/odd basis serves forward one-element Dirichlet solves");
-    if (a.adjoint) {
+    if (a.eo == general) return set_error(HZ_ERR_ARG, "DIAG form: the even/odd basis is for one Dirichlet element");
+    if (a.adjoint && a.eo) {
+        hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true, true>), dim3(blocks), dim3(256), 0, s, a);
+    } else if (a.adjoint) {
         hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, false, true, false>), dim3(blocks), dim3(256), 0, s, a);
     } else if (general) {
         hipLaunchKernelGGL((diag_modes_kernel<NJ, RT, true, false, false>), dim3(blocks), dim3(256), 0, s, a);

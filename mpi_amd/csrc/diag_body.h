@@ -25,8 +25,7 @@
 //   fbar_int = V^{-T} ((y + P0b r0 + PNb rN) * inv),  fbar = 0 at the ends.
 // The kernel is the forward one with B1 = V, B2 = V^{-1}.
 //
-// Even/odd instance (EO: forward solves of one element with Dirichlet
-// faces).  The GLL nodes mirror about the element centre, so Dzz_int
+// Even/odd instance (EO: one element with Dirichlet faces).  The GLL nodes mirror about the element centre, so Dzz_int
 // commutes with the reversal J up to the nodes' rounding; with the fold
 // F f = (f_i + f_{n-1-i} | f_i - f_{n-1-i}) the symmetrised operator
 // (A + JAJ)/2 is block diagonal and is diagonalised block by block
@@ -36,7 +35,9 @@
 // k range: half the matrix work.  The tile is folded while it is staged and
 // unfolded through LDS before the store; between the contractions
 // z -= (M z) / (mu - sigma) (M = Winv F (A - JAJ)/2 F^{-1} W, bf16 operands,
-// FP32 sums) corrects to first order from the symmetrised operator to A.
+// FP32 sums) corrects to first order from the symmetrised operator to A;
+// the adjoint applies the transposed correction q -= (M^T q) / (mu - sigma)
+// to q = y / (mu - sigma) before the end-value dots.
 //
 // A work-group owns RT tile rows (spectral column x element) and all NP
 // (padded n) eigen/grid indices.  Contractions are transposed so that MFMA
@@ -183,12 +184,15 @@ struct DiagShared {
 
 // Tile `bx` (RT rows) of the diagonalised mode solve; 256 threads.  Starts
 // with a barrier so that back-to-back tiles can reuse the LDS.  EO: the
-// even/odd eigenbasis (a.eo; forward solves of one element with Dirichlet
-// faces), whose contractions run over half the k range.
+// even/odd eigenbasis (a.eo; one element with Dirichlet faces), whose
+// contractions run over half the k range.
 template <int NJ, int RT, bool GENERAL, bool ADJ, bool EO = !GENERAL && !ADJ>
 __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<NJ, RT, GENERAL, ADJ> &sh)
 {
-    static_assert(!(EO && (GENERAL || ADJ)), "the even/odd basis serves forward one-element Dirichlet solves");
+    static_assert(!(EO && GENERAL), "the even/odd basis serves one element with Dirichlet faces");
+    // EO adjoint: q = y / (mu - sigma) is corrected before the end-value
+    // dots use it, so acc holds q (not y) from there on
+    constexpr bool QIN = EO && ADJ;
     constexpr int NP = 16 * NJ;      // padded interior size
     constexpr int LD = NP + 2;       // LDS row stride (doubles)
     constexpr int H = NP / 2;        // EO: the odd coordinates' offset
@@ -296,6 +300,44 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
     const int li = lane % 16, lk = lane / 16;
     tile_gemm<NP, LD, RBW, JT, false, KN>(acc, tile + kb, a.VinvT + (long long)kb * NP, c0, r0, li, lk);
     __syncthreads();  // every wave has read the F tile
+    if constexpr (QIN) {
+        // transposed first-order correction, q -= (M^T q) / (mu - sigma)
+        // (a.eoM holds M^T's operand): q into the tile, the bf16
+        // contraction over the other half, the update in registers
+        double iq[RBW][JT][4];
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                const int j = 16 * (c0 + jt) + li;
+                const bool jv = j < H ? j < he : j - H < ho;
+                const double mu = jv ? a.mu[j] : 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * (r0 + rb) + 4 * r + lk;
+                    iq[rb][jt][r] = jv ? 1.0 / (mu - row_sig[row]) : 0.0;
+                    acc[rb][jt][r] *= iq[rb][jt][r];
+                    tile[row * LD + j] = acc[rb][jt][r];
+                }
+            }
+        __syncthreads();
+        f4 cacc[RBW][JT];
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) cacc[rb][jt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        const int ko = H - kb;
+        tile_gemm_bf16_rows<NP, LD, RBW, JT, H>(cacc, tile + ko, reinterpret_cast<const s4 *>(a.eoM) + ko / 4 * NP,
+                                                c0, r0, li, lk);
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[rb][jt][r] = fma(-(double)cacc[rb][jt][r], iq[rb][jt][r], acc[rb][jt][r]);
+        // the tile is next written after the barrier that follows the dots
+    }
 
     // ---- shift, interface sweep, assemble Z^T in the tile ----
     // q = y / (mu - sigma) stays in registers; GENERAL needs r0|rN . q per
@@ -322,7 +364,7 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
                 const double iv = jv ? 1.0 / d : 0.0;
                 inv[rb][jt][r] = iv;
                 if (GENERAL || ADJ) {
-                    const double q = acc[rb][jt][r] * iv;
+                    const double q = QIN ? acc[rb][jt][r] : acc[rb][jt][r] * iv;
                     p0[r] = fma(v0, q, p0[r]);
                     pN[r] = fma(vN, q, pN[r]);
                 }
@@ -467,11 +509,12 @@ __device__ __forceinline__ void diag_tile(const DiagArgs &a, int bx, DiagShared<
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * (r0 + rb) + 4 * r + lk;
                 const double h = fma(row_zl[row], w0, row_zr[row] * wN);
-                tile[row * LD + j] = (acc[rb][jt][r] + h) * inv[rb][jt][r];
+                tile[row * LD + j] =
+                    QIN ? fma(h, inv[rb][jt][r], acc[rb][jt][r]) : (acc[rb][jt][r] + h) * inv[rb][jt][r];
             }
         }
     __syncthreads();
-    if constexpr (EO) {
+    if constexpr (EO && !ADJ) {
         // first-order correction from the symmetrised operator to the
         // reference's: z -= (M z) / (mu - sigma), M z from the other half's
         // coordinates (M maps even to odd and back), bf16 operands with FP32

@@ -102,9 +102,10 @@ struct DiagArgs {
     const double *hom;          // [modes][4]: Dz0.ul, DzN.ul, Dz0.ur, DzN.ur
     double alpha_l, beta_l, alpha_r, beta_r;
     int adjoint;                // transposed map; VinvT <- V, VT <- Vinv (row-major)
-    int eo;                     // even/odd eigenbasis (forward, one element, Dirichlet faces):
-                                // the vectors padded to NP, the operands block diagonal
-    const short *eoM;           // eo: the correction to the reference's operator, bf16 [NP/4][NP][4]
+    int eo;                     // even/odd eigenbasis (one element, Dirichlet faces): the
+                                // vectors padded to NP, the operands block diagonal
+    const short *eoM;           // eo: the correction to the reference's operator (M, or M^T for
+                                // the adjoint), bf16 [NP/4][NP][4]
     double *gbar_re, *gbar_im;  // adjoint only: [side][nx][M] face gradients
     int rt;                     // tile rows (spectral columns x planes x elements); 0 = default
     int periodic;               // z periodic: cyclic interface system, no face data
@@ -227,11 +228,12 @@ struct hz_plan {
     double *d_mu = nullptr, *d_w0 = nullptr, *d_wN = nullptr, *d_r0 = nullptr, *d_rN = nullptr;
     double *d_VinvT = nullptr, *d_VT = nullptr, *d_hom = nullptr;
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
-    // even/odd eigenbasis (forward solves of one element with Dirichlet
-    // faces): mu, w0, wN, r0, rN padded to NP each; operands NP x NP
+    // even/odd eigenbasis (one element with Dirichlet faces): mu, w0, wN,
+    // r0, rN padded to NP each; operands NP x NP
     std::vector<double> eo_vec;
     double *d_eo_vec = nullptr, *d_eo_B1 = nullptr, *d_eo_B2 = nullptr, *d_eo_hom = nullptr;
-    short *d_eo_M = nullptr;  // first-order correction to the reference's operator (bf16)
+    double *d_eo_B1a = nullptr, *d_eo_B2a = nullptr;  // adjoint operands
+    short *d_eo_M = nullptr, *d_eo_Ma = nullptr;      // first-order correction (bf16), forward and adjoint
     int diag_rt = 0;                                  // autotuned tile rows
     int fft_w[4] = {0, 0, 0, 0};                      // autotuned lanes per pass
     double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass

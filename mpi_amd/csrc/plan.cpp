@@ -77,10 +77,11 @@ void free_plan(hz_plan *p)
     (void)hipFree(p->spec.re);  // one allocation holds both planes
     (void)hipFree(p->bc.re);
     for (double *q : {p->d_mu, p->d_w0, p->d_wN, p->d_r0, p->d_rN, p->d_VinvT, p->d_VT, p->d_hom, p->d_Vrow,
-                      p->d_Vinvrow, p->d_eo_vec, p->d_eo_B1, p->d_eo_B2, p->d_eo_hom, p->d_partial, p->d_DzzTP,
-                      p->d_UL, p->d_UR, p->d_hist, p->d_closure})
+                      p->d_Vinvrow, p->d_eo_vec, p->d_eo_B1, p->d_eo_B2, p->d_eo_B1a, p->d_eo_B2a, p->d_eo_hom,
+                      p->d_partial, p->d_DzzTP, p->d_UL, p->d_UR, p->d_hist, p->d_closure})
         (void)hipFree(q);
     (void)hipFree(p->d_eo_M);
+    (void)hipFree(p->d_eo_Ma);
     (void)hipFree(p->d_VinvT32);
     (void)hipFree(p->d_VT32);
     (void)hipFree(p->d_xspec);
@@ -205,19 +206,20 @@ int compute_shifts(hz_plan *p)
 // anticommutes with J).  The kernel applies F and F^-1 as sums and
 // differences of mirrored entries (the halves of F^-1 are folded into D W,
 // D = 1/2 except on the centre row of an odd n).  Operands, NP x NP
-// row-major: B1 = Winv^T, B2 = (D W)^T (FP64), M in bf16 (correction_operator); r0|rN =
+// row-major: B1 = Winv^T, B2 = (D W)^T, adjoint B1 = D W, B2 = Winv (FP64),
+// M and M^T in bf16 (correction_operator); r0|rN =
 // Dz[0|N-1][1..N-2] F^-1 W (the homogeneous coefficients' basis).
-std::vector<short> correction_operator(const std::vector<double> &A, int n, int NP, const std::vector<double> &W,
-                                       const std::vector<double> &Wi);
+std::array<std::vector<short>, 2> correction_operator(const std::vector<double> &A, int n, int NP,
+                                                      const std::vector<double> &W, const std::vector<double> &Wi);
 
 int setup_even_odd(hz_plan *p, const std::vector<double> &A)
 {
     const int n = p->n, N = p->N, NP = p->NP, H = NP / 2, m = n / 2;
     static std::mutex cache_mu;
     static std::map<std::pair<std::vector<double>, int>, std::array<std::vector<double>, 3>> cache;
-    static std::map<std::pair<std::vector<double>, int>, std::vector<short>> cache_m;
+    static std::map<std::pair<std::vector<double>, int>, std::array<std::vector<short>, 2>> cache_m;
     std::vector<double> mu, W, Wi;
-    std::vector<short> Mq;
+    std::array<std::vector<short>, 2> Mq;
     const auto key = std::make_pair(A, NP);
     {
         std::lock_guard<std::mutex> lk(cache_mu);
@@ -275,15 +277,18 @@ int setup_even_odd(hz_plan *p, const std::vector<double> &A)
         v[3 * NP + j] = (double)t0;
         v[4 * NP + j] = (double)tN;
     }
-    std::vector<double> B1((size_t)NP * NP), B2((size_t)NP * NP);
+    std::vector<double> B1((size_t)NP * NP), B2((size_t)NP * NP), B1a((size_t)NP * NP), B2a((size_t)NP * NP);
     for (int r = 0; r < NP; ++r) {
         const double dr = (n % 2 && r == m) ? 1.0 : 0.5;
         for (int c = 0; c < NP; ++c) {
+            const double dw = dr * W[(size_t)r * NP + c];  // (D W)[r][c]
             B1[(size_t)c * NP + r] = Wi[(size_t)r * NP + c];
-            B2[(size_t)c * NP + r] = dr * W[(size_t)r * NP + c];  // (D W)[r][c]
+            B2[(size_t)c * NP + r] = dw;
+            B1a[(size_t)r * NP + c] = dw;
+            B2a[(size_t)r * NP + c] = Wi[(size_t)r * NP + c];
         }
     }
-    if (Mq.empty()) {
+    if (Mq[0].empty()) {
         Mq = correction_operator(A, n, NP, W, Wi);
         std::lock_guard<std::mutex> lk(cache_mu);
         if (cache.size() < 256) {
@@ -293,7 +298,8 @@ int setup_even_odd(hz_plan *p, const std::vector<double> &A)
     }
     int st;
     if ((st = upload(&p->d_eo_vec, p->eo_vec)) || (st = upload(&p->d_eo_B1, B1)) || (st = upload(&p->d_eo_B2, B2)) ||
-        (st = upload(&p->d_eo_M, Mq)))
+        (st = upload(&p->d_eo_B1a, B1a)) || (st = upload(&p->d_eo_B2a, B2a)) || (st = upload(&p->d_eo_M, Mq[0])) ||
+        (st = upload(&p->d_eo_Ma, Mq[1])))
         return st;
     return HZ_OK;
 }
@@ -301,16 +307,19 @@ int setup_even_odd(hz_plan *p, const std::vector<double> &A)
 // The even/odd solve's first-order correction M = Winv (F Ea F^-1) W, Ea =
 // (A - JAJ)/2 (the middle factor column by column: F^-1 of a unit vector has
 // one or two entries; then the two products, in long double), as the bf16
-// B operand of c = M z: Mq[(k / 4) * NP + j][k % 4] = bf16(M[j][k]),
+// B operands of c = M z (forward) and c = M^T q (adjoint):
+// [0][(k / 4) * NP + j][k % 4] = bf16(M[j][k]), [1][...] = bf16(M[k][j]),
 // rounded to nearest even.
-std::vector<short> correction_operator(const std::vector<double> &A, int n, int NP, const std::vector<double> &W,
-                                       const std::vector<double> &Wi)
+std::array<std::vector<short>, 2> correction_operator(const std::vector<double> &A, int n, int NP,
+                                                      const std::vector<double> &W, const std::vector<double> &Wi)
 {
     const int H = NP / 2, m = n / 2;
     typedef long double ld;
     std::vector<ld> Et((size_t)NP * NP, 0), T((size_t)NP * NP, 0);
     std::vector<ld> y(n);
-    auto ea = [&](int r, int c) { return ((ld)A[(size_t)r * n + c] - (ld)A[(size_t)(n - 1 - r) * n + (n - 1 - c)]) / 2; };
+    auto ea = [&](int r, int c) {
+        return ((ld)A[(size_t)r * n + c] - (ld)A[(size_t)(n - 1 - r) * n + (n - 1 - c)]) / 2;
+    };
     for (int c = 0; c < NP; ++c) {
         const bool even = c < H;
         const int j = even ? c : c - H;
@@ -329,7 +338,9 @@ std::vector<short> correction_operator(const std::vector<double> &A, int n, int 
             if (e == 0) continue;
             for (int c = 0; c < NP; ++c) T[(size_t)r * NP + c] += e * (ld)W[(size_t)k * NP + c];
         }
-    std::vector<short> Mq((size_t)NP * NP, 0);
+    std::array<std::vector<short>, 2> Mq;
+    Mq[0].assign((size_t)NP * NP, 0);
+    Mq[1].assign((size_t)NP * NP, 0);
     for (int r = 0; r < NP; ++r)      // M = Winv T
         for (int c = 0; c < NP; ++c) {
             ld acc = 0;
@@ -338,7 +349,8 @@ std::vector<short> correction_operator(const std::vector<double> &A, int n, int 
             uint32_t u;
             std::memcpy(&u, &f, 4);
             u += 0x7FFFu + ((u >> 16) & 1u);
-            Mq[((size_t)(c / 4) * NP + r) * 4 + c % 4] = (short)(u >> 16);
+            Mq[0][((size_t)(c / 4) * NP + r) * 4 + c % 4] = (short)(u >> 16);  // B[k = c][j = r]
+            Mq[1][((size_t)(r / 4) * NP + c) * 4 + r % 4] = (short)(u >> 16);  // B[k = r][j = c]
         }
     return Mq;
 }
@@ -556,22 +568,21 @@ int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipSt
 }  // namespace
 
 // Kernel arguments of the z-diagonalised mode solve (forward map).
-// The diag kernel's operands: the even/odd eigenbasis for the forward solve
-// of one element with Dirichlet faces (the kernels' EO instances), the full
-// eigenbasis otherwise.
+// The diag kernel's operands: the even/odd eigenbasis for one element with
+// Dirichlet faces (the kernels' EO instances), the full eigenbasis otherwise.
 void hz::diag_basis(const hz_plan *p, DiagArgs &a, bool adjoint)
 {
     const int NP = p->NP;
-    a.eo = !adjoint && !diag_general(p);
-    a.eoM = p->d_eo_M;
+    a.eo = !diag_general(p);
+    a.eoM = adjoint ? p->d_eo_Ma : p->d_eo_M;
     if (a.eo) {
         a.mu = p->d_eo_vec;
         a.w0 = p->d_eo_vec + NP;
         a.wN = p->d_eo_vec + 2 * NP;
         a.r0 = p->d_eo_vec + 3 * NP;
         a.rN = p->d_eo_vec + 4 * NP;
-        a.VinvT = p->d_eo_B1;
-        a.VT = p->d_eo_B2;
+        a.VinvT = adjoint ? p->d_eo_B1a : p->d_eo_B1;
+        a.VT = adjoint ? p->d_eo_B2a : p->d_eo_B2;
         a.hom = p->d_eo_hom;
     } else {
         a.mu = p->d_mu;
