@@ -167,6 +167,14 @@ int hz_adjoint_noalloc(hz_plan *plan, const double *ubar, const double *u, doubl
  * pass on the caller's stream.  mpi_amd/allen_cahn.py calls it with
  * a3 = 1/eps^2, a1 = -(1/eps^2 + lambda). */
 int hz_cubic_rhs(const double *u, double *f, long long n, double a3, double a1, hz_stream_t stream);
+/* hz_solve with the right-hand side f = a3 u^3 + a1 u formed inside the
+ * forward transform's loads (the operations of hz_cubic_rhs, so the result
+ * equals hz_cubic_rhs followed by hz_solve bit for bit), without the
+ * intermediate field: a semi-implicit Allen-Cahn step in one call.  u is
+ * read only by the first pass, so out may be u.  No counterpart in the
+ * reference. */
+int hz_solve_cubic(hz_plan *plan, const double *u, double a3, double a1, const double *g_l,
+                   const double *g_r, double *out, hz_stream_t stream);
 
 /* Replaces FirstDerivativeZ / SecondDerivativeZ (NonPeriodic3D_MPI.cpp:632-683):
  * out = Dz u (order 1) or Dzz u (order 2) applied to every element column.

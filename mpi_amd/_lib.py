@@ -82,6 +82,7 @@ SIGNATURES = {
     "hz_probe_ceilings": (ctypes.c_int, [_PD, _PD, ctypes.c_size_t]),
     "hz_plan_periodic_closure": (ctypes.c_int, [_VP, _PD]),
     "hz_cubic_rhs": (ctypes.c_int, [_VP, _VP, ctypes.c_longlong, _D, _D, _VP]),
+    "hz_solve_cubic": (ctypes.c_int, [_VP, _VP, _D, _D, _VP, _VP, _VP, _VP]),
 }
 
 _lib = None

@@ -32,7 +32,6 @@ import math
 import numpy as np
 import torch
 
-from ._lib import check
 from .solver import HelmholtzPlan
 
 
@@ -74,13 +73,9 @@ class AllenCahn:
 
     def step(self, u, eps, lam, out=None):
         # f = (u^3 - u) / eps^2 - lam u = c u^3 - (c + lam) u, c = 1 / eps^2,
-        # in one pass (hz_cubic_rhs)
+        # formed inside the solve's forward transform (hz_solve_cubic)
         c = 1.0 / (eps * eps)
-        u = u.contiguous()
-        f = torch.empty_like(u)
-        check(self.plan.lib.hz_cubic_rhs(u.data_ptr(), f.data_ptr(), u.numel(), c, -(c + lam),
-                                         torch.cuda.current_stream().cuda_stream))
-        return self.plan.solve(f, self.g0, self.g0, out=out)
+        return self.plan.solve_cubic(u.contiguous(), c, -(c + lam), self.g0, self.g0, out=out)
 
     def run(self, u0, eps):
         """u_T after `steps` steps (no checkpoints kept)."""

@@ -95,11 +95,21 @@ __device__ __forceinline__ void dft_regs(double (&re)[R], double (&im)[R], const
 // Addressing of a tile's operands.  Point p of the thread's line is at
 // element index off(p) = base_offset + p * point_stride of each operand.
 // Addr64: per-thread pointers and 64-bit indices (any lane mix, any size).
+// A lane with `cub` set reads x * (c3 x^2 + c1) for the real input x (the
+// cubic right-hand side hz_solve_cubic forms inside the R2C pass; the same
+// operations as hz_cubic_rhs).
+__device__ __forceinline__ double cubic_map(double x, bool cub, double c3, double c1)
+{
+    return cub ? x * fma(c3 * x, x, c1) : x;
+}
+
 struct Addr64 {
     gcdouble *ire, *iim;
     gdouble *ore, *oim;
     long long ib, ips, ob, ops;
-    __device__ __forceinline__ double ldr(int p) const { return ire[ib + (long long)p * ips]; }
+    bool cub;
+    double c3, c1;
+    __device__ __forceinline__ double ldr(int p) const { return cubic_map(ire[ib + (long long)p * ips], cub, c3, c1); }
     __device__ __forceinline__ double ldi(int p) const { return iim[ib + (long long)p * ips]; }
     __device__ __forceinline__ void str(int p, double v) const { ore[ob + (long long)p * ops] = v; }
     __device__ __forceinline__ void sti(int p, double v) const { oim[ob + (long long)p * ops] = v; }
@@ -111,13 +121,15 @@ struct Addr32 {
     gcdouble *ire, *iim;
     gdouble *ore, *oim;
     unsigned ib, ips, ob, ops;
+    bool cub;
+    double c3, c1;
     template <class P>
     __device__ __forceinline__ static P at(P base, unsigned idx)
     {
         typedef __attribute__((address_space(1))) char gchar;
         return (P)((gchar *)base + 8u * idx);
     }
-    __device__ __forceinline__ double ldr(int p) const { return *at(ire, ib + (unsigned)p * ips); }
+    __device__ __forceinline__ double ldr(int p) const { return cubic_map(*at(ire, ib + (unsigned)p * ips), cub, c3, c1); }
     __device__ __forceinline__ double ldi(int p) const { return *at(iim, ib + (unsigned)p * ips); }
     __device__ __forceinline__ void str(int p, double v) const { *at(ore, ob + (unsigned)p * ops) = v; }
     __device__ __forceinline__ void sti(int p, double v) const { *at(oim, ob + (unsigned)p * ops) = v; }
@@ -495,6 +507,9 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
     ad.ips = xl ? sval(a.x_in_point_stride) : sval(a.in_point_stride);
     ad.ob = xl ? (long long)batch * sval(a.x_out_batch_stride) : (long long)batch * sval(a.out_batch_stride) + lane;
     ad.ops = xl ? sval(a.x_out_point_stride) : sval(a.out_point_stride);
+    ad.cub = KIND == FFT_R2C && a.pre_cubic && !xl;
+    ad.c3 = a.pre_c3;
+    ad.c1 = a.pre_c1;
     fft_core_any<L, R1, R2, RB, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
 }
 
@@ -546,6 +561,7 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ob = w;
         ad.ips = (unsigned)a.in_point_stride;
         ad.ops = (unsigned)a.out_point_stride;
+        ad.cub = KIND == FFT_R2C && a.pre_cubic;
         live = l0 + w < lanes;
     } else {
         const int nbb = (nb + W - 1) / W;
@@ -561,8 +577,11 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ob = (unsigned)(w * a.x_out_batch_stride);
         ad.ips = (unsigned)a.x_in_point_stride;
         ad.ops = (unsigned)a.x_out_point_stride;
+        ad.cub = false;
         live = b0 + w < nb;
     }
+    ad.c3 = a.pre_c3;
+    ad.c1 = a.pre_c1;
     fft_core_any<L, R1, R2, RB, W, KIND>(ad, live, a.tw, tile);
 }
 
@@ -593,7 +612,7 @@ __device__ __forceinline__ void unit_line_element(const FftArgs &a, long long t)
         ore = a.xout_re[sx] + oo;
         if (KIND != FFT_C2R) oim = a.xout_im[sx] + oo;
     }
-    const double re = *ire;
+    const double re = cubic_map(*ire, KIND == FFT_R2C && a.pre_cubic && l < a.lanes, a.pre_c3, a.pre_c1);
     const double im = (KIND == FFT_C2C_FWD || KIND == FFT_C2C_INV) ? *iim : 0.0;
     *ore = re;
     if (KIND != FFT_C2R) *oim = im;

@@ -62,6 +62,9 @@ struct FftArgs {
     double *xout_re[2], *xout_im[2];
     long long x_in_batch_stride, x_in_point_stride, x_out_batch_stride, x_out_point_stride;
     int w;                         // z lanes per work-group (0 = default for L)
+    // R2C only: the main lanes read x (pre_c3 x^2 + pre_c1) for the input x
+    int pre_cubic;
+    double pre_c3, pre_c1;
 };
 int launch_fft(int L, FftKind kind, const FftArgs &a, hipStream_t s);
 // Lane widths offered for length L (autotuned per plan); returns the count.

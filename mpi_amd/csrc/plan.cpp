@@ -543,11 +543,19 @@ namespace {
 // Forward x-y transform of a [nx][ny][lanes] field into split half spectra;
 // with gl/gr the two face planes ride along as extra lanes of the same
 // launches and land in the boundary spectra.
+// Forward transform; with `cubic` the R2C pass reads in * (c[0] in^2 + c[1])
+// (hz_solve_cubic's right-hand side).
 int forward_xy(hz_plan *p, const double *in, double *re, double *im, int lanes, hipStream_t s,
-               const double *gl = nullptr, const double *gr = nullptr, long long xs = 0)
+               const double *gl = nullptr, const double *gr = nullptr, long long xs = 0,
+               const double *cubic = nullptr)
 {
-    int st =
-        launch_fft(p->ny, FFT_R2C, xy_pass_args(p, 0, in, re, im, nullptr, lanes, gl, gr, nullptr, nullptr, xs), s);
+    FftArgs r2c = xy_pass_args(p, 0, in, re, im, nullptr, lanes, gl, gr, nullptr, nullptr, xs);
+    if (cubic) {
+        r2c.pre_cubic = 1;
+        r2c.pre_c3 = cubic[0];
+        r2c.pre_c1 = cubic[1];
+    }
+    int st = launch_fft(p->ny, FFT_R2C, r2c, s);
     if (st) return st;
     return launch_fft(p->nx, FFT_C2C_FWD,
                       xy_pass_args(p, 1, nullptr, re, im, nullptr, lanes, gl, gr, nullptr, nullptr, xs), s);
@@ -567,7 +575,6 @@ int inverse_xy(hz_plan *p, double *re, double *im, double *out, int lanes, hipSt
 
 }  // namespace
 
-// Kernel arguments of the z-diagonalised mode solve (forward map).
 // The diag kernel's operands: the even/odd eigenbasis for one element with
 // Dirichlet faces (the kernels' EO instances), the full eigenbasis otherwise.
 void hz::diag_basis(const hz_plan *p, DiagArgs &a, bool adjoint)
@@ -601,6 +608,7 @@ bool hz::diag_general(const hz_plan *p)
     return p->ne > 1 || p->d.alpha_l != 0.0 || p->d.alpha_r != 0.0 || p->d.periodic_z;
 }
 
+// Kernel arguments of the z-diagonalised mode solve (forward map).
 DiagArgs hz::diag_args(const hz_plan *p)
 {
     DiagArgs a{};
@@ -908,7 +916,8 @@ int hz_plan_set_robin(hz_plan *p, double alpha_l, double beta_l, double alpha_r,
 }
 
 static int solve_impl(hz_plan *p, const double *f, const double *gl, const double *gr,
-                      const double *op_diag, const double *op_full, double *u, hipStream_t s)
+                      const double *op_diag, const double *op_full, double *u, hipStream_t s,
+                      const double *cubic = nullptr)
 {
     if (!p || !f || !u) return set_error(HZ_ERR_ARG, "null argument");
     if (p->d.periodic_z) {
@@ -917,7 +926,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
         return set_error(HZ_ERR_ARG, "null face data");
     }
     StageRecorder rec(p, s);
-    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr, p->xs);
+    int st = forward_xy(p, f, p->spec.re, p->spec.im, p->K, s, gl, gr, p->xs, cubic);
     rec.mark("fft_forward");
     if (st) return st;
     if (p->d.form == HZ_FORM_PLAIN_LU || op_diag || op_full) {
@@ -1007,6 +1016,13 @@ int hz_solve(hz_plan *p, const double *f, const double *gl, const double *gr, do
              hz_stream_t stream)
 {
     return solve_impl(p, f, gl, gr, nullptr, nullptr, u, (hipStream_t)stream);
+}
+
+int hz_solve_cubic(hz_plan *p, const double *u, double a3, double a1, const double *gl, const double *gr,
+                   double *out, hz_stream_t stream)
+{
+    const double c[2] = {a3, a1};
+    return solve_impl(p, u, gl, gr, nullptr, nullptr, out, (hipStream_t)stream, c);
 }
 
 int hz_solve_operators(hz_plan *p, const double *f, const double *gl, const double *gr,

@@ -98,6 +98,19 @@ class HelmholtzPlan:
         check(self.lib.hz_solve(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(u), _stream_ptr(stream)))
         return u
 
+    def solve_cubic(self, u, a3, a1, gl=None, gr=None, out=None, stream=None):
+        """solve() with the right-hand side a3 u^3 + a1 u formed inside the
+        forward transform (hz_solve_cubic; equal to solve(cubic_rhs(u)) bit for
+        bit)."""
+        self._check_field(u, (self.nx, self.ny, self.nz))
+        if not self.desc.periodic_z:
+            self._check_field(gl, (self.nx, self.ny))
+            self._check_field(gr, (self.nx, self.ny))
+        v = out if out is not None else self.empty_field()
+        check(self.lib.hz_solve_cubic(self.handle, _ptr(u), float(a3), float(a1), _ptr(gl), _ptr(gr), _ptr(v),
+                                      _stream_ptr(stream)))
+        return v
+
     def solve_operators(self, f, gl, gr, op_diag=None, op_full=None, out=None, stream=None):
         u = out if out is not None else self.empty_field()
         check(self.lib.hz_solve_operators(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(op_diag),

@@ -404,6 +404,40 @@ def test_cubic_rhs_matches_torch(cuda, n):
         _lib.check(lib.hz_cubic_rhs(None, f.data_ptr(), 5, a3, a1, s))
 
 
+@pytest.mark.parametrize("shape,form,bc", [((64, 32, 33, 1), "diag", "neumann"), ((32, 64, 17, 2), "diag", "robin"),
+                                           ((64, 1, 33, 1), "diag", "dirichlet"), ((1, 1, 65, 1), "diag", "neumann"),
+                                           ((16, 16, 17, 1), "plain", "dirichlet"),
+                                           ((32, 32, 33, 1), "mixed", "neumann")])
+def test_solve_cubic_equals_rhs_then_solve(cuda, shape, form, bc):
+    """hz_solve_cubic (the right-hand side a3 u^3 + a1 u formed inside the
+    forward transform's loads) equals hz_cubic_rhs followed by hz_solve bit
+    for bit, on 3-D, 2-D (a unit y axis: the copy pass) and 1-D plans, every
+    form; out may be u itself."""
+    import torch
+    from mpi_amd import _lib
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny, N, ne = shape
+    ab = {"neumann": (1.0, 0.0, 1.0, 0.0), "robin": (0.7, 1.3, 0.4, 2.0), "dirichlet": (0.0, 1.0, 0.0, 1.0)}[bc]
+    plan = HelmholtzPlan(nx, ny, N, ne, lam=2.5e3, alpha_l=ab[0], beta_l=ab[1], alpha_r=ab[2], beta_r=ab[3],
+                         form=form)
+    g = torch.Generator(device="cuda").manual_seed(nx * ny + N)
+    u = torch.rand((nx, ny, plan.nz), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    gl = torch.rand((nx, ny), dtype=torch.float64, device="cuda", generator=g)
+    gr = torch.rand((nx, ny), dtype=torch.float64, device="cuda", generator=g)
+    a3, a1 = 2500.0, -(2500.0 + 2.5e3)
+    lib = _lib.load()
+    f = torch.empty_like(u)
+    _lib.check(lib.hz_cubic_rhs(u.data_ptr(), f.data_ptr(), u.numel(), a3, a1,
+                                torch.cuda.current_stream().cuda_stream))
+    want = plan.solve(f, gl, gr)
+    got = plan.solve_cubic(u, a3, a1, gl, gr)
+    assert torch.equal(got, want)
+    v = u.clone()
+    plan.solve_cubic(v, a3, a1, gl, gr, out=v)
+    assert torch.equal(v, want)
+    plan.close()
+
+
 def test_allen_cahn_graph_refuses_stale_eps(cuda):
     """A captured forward holds its eps in the right-hand side's
     coefficients: after the plan's lambda changes, replay() refuses."""
