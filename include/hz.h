@@ -175,6 +175,16 @@ int hz_cubic_rhs(const double *u, double *f, long long n, double a3, double a1, 
  * reference. */
 int hz_solve_cubic(hz_plan *plan, const double *u, double a3, double a1, const double *g_l,
                    const double *g_r, double *out, hz_stream_t stream);
+/* One backward step through hz_solve_cubic's right-hand side, fused: with
+ * fbar = dL/df of the step u_in -> u_out (from hz_adjoint),
+ *   ubar = fbar (3 a3 u_in^2 + a1)          (dL/du_in; ubar may be fbar)
+ *   *acc += sum fbar (b1 (u_out - u_in) + b3 (u_in^3 - u_in))
+ * (acc a device double; a parameter's gradient through the step, e.g. the
+ * Allen-Cahn eps with b1 = dlambda/deps, b3 = da3/deps).  Fields of the
+ * plan's size, device pointers; the sum runs in a fixed order (the same on
+ * every run).  No counterpart in the reference. */
+int hz_cubic_adjoint(hz_plan *plan, const double *fbar, const double *u_in, const double *u_out, double a3,
+                     double a1, double b1, double b3, double *ubar, double *acc, hz_stream_t stream);
 
 /* Replaces FirstDerivativeZ / SecondDerivativeZ (NonPeriodic3D_MPI.cpp:632-683):
  * out = Dz u (order 1) or Dzz u (order 2) applied to every element column.

@@ -83,6 +83,7 @@ SIGNATURES = {
     "hz_plan_periodic_closure": (ctypes.c_int, [_VP, _PD]),
     "hz_cubic_rhs": (ctypes.c_int, [_VP, _VP, ctypes.c_longlong, _D, _D, _VP]),
     "hz_solve_cubic": (ctypes.c_int, [_VP, _VP, _D, _D, _VP, _VP, _VP, _VP]),
+    "hz_cubic_adjoint": (ctypes.c_int, [_VP, _VP, _VP, _VP, _D, _D, _D, _D, _VP, _VP, _VP]),
 }
 
 _lib = None

@@ -168,6 +168,9 @@ int launch_spectral_derivative(double *re, double *im, long long A, long long B,
 // partial >= kDotPartials doubles (the first pass runs at most that many work-groups)
 constexpr int kDotPartials = 1024;
 int launch_dot(const double *a, const double *b, long long n, double *partial, double *out, hipStream_t s);
+// ubar = fbar (3 a3 u_in^2 + a1); *acc += sum fbar (b1 (u_out - u_in) + b3 (u_in^3 - u_in))
+int launch_cubic_adjoint(const double *fbar, const double *u_in, const double *u_out, long long n, double a3,
+                         double a1, double b1, double b3, double *ubar, double *partial, double *acc, hipStream_t s);
 // n pseudo-random doubles in [-0.5, 0.5) (a hash of the index and seed):
 // timing data for the plan-time tuners.
 int launch_fill_noise(double *p, size_t n, unsigned long long seed, hipStream_t s);

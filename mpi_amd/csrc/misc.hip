@@ -59,6 +59,47 @@ __global__ __launch_bounds__(256) void cubic_kernel(const double *__restrict__ u
     f[i] = x * fma(a3 * x, x, a1);
 }
 
+// One backward step of the semi-implicit cubic right-hand side f(u) =
+// a3 u^3 + a1 u: ubar = fbar f'(u_in) = fbar (3 a3 u_in^2 + a1) (ubar may be
+// fbar), and the work-group's partial sum of fbar (b1 (u_out - u_in) +
+// b3 (u_in^3 - u_in)) into partial[blockIdx.x] (a fixed grid and order, so
+// the sum is the same on every run).
+__global__ __launch_bounds__(256) void cubic_adjoint_kernel(const double *fbar, const double *__restrict__ u_in,
+                                                            const double *__restrict__ u_out, long long n, double a3,
+                                                            double a1, double b1, double b3, double *ubar,
+                                                            double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double fb = fbar[i], x = u_in[i], x2 = x * x;
+        acc = fma(fb, b1 * (u_out[i] - x) + b3 * (x2 * x - x), acc);
+        ubar[i] = fb * fma(3.0 * a3, x2, a1);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// *acc += the sum of partial[0 .. np) (one work-group, fixed order).
+__global__ __launch_bounds__(256) void accumulate_final_kernel(const double *__restrict__ partial, int np, double *acc)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += partial[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += red[0];
+}
+
 __global__ __launch_bounds__(256) void dot_final_kernel(const double *__restrict__ partial, int np, double *out)
 {
     __shared__ double red[256];
@@ -120,6 +161,17 @@ int launch_dot(const double *a, const double *b, long long n, double *partial, d
     const int blocks = (int)std::min<long long>(kDotPartials, std::max<long long>(1, (n + 255) / 256));
     hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(256), 0, s, a, b, n, partial);
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, partial, blocks, out);
+    HZ_HIP(hipGetLastError());
+    return HZ_OK;
+}
+
+int launch_cubic_adjoint(const double *fbar, const double *u_in, const double *u_out, long long n, double a3,
+                         double a1, double b1, double b3, double *ubar, double *partial, double *acc, hipStream_t s)
+{
+    const int blocks = (int)std::min<long long>(kDotPartials, std::max<long long>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(cubic_adjoint_kernel, dim3(blocks), dim3(256), 0, s, fbar, u_in, u_out, n, a3, a1, b1, b3, ubar,
+                       partial);
+    hipLaunchKernelGGL(accumulate_final_kernel, dim3(1), dim3(256), 0, s, partial, blocks, acc);
     HZ_HIP(hipGetLastError());
     return HZ_OK;
 }

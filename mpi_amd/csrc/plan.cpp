@@ -1025,6 +1025,16 @@ int hz_solve_cubic(hz_plan *p, const double *u, double a3, double a1, const doub
     return solve_impl(p, u, gl, gr, nullptr, nullptr, out, (hipStream_t)stream, c);
 }
 
+int hz_cubic_adjoint(hz_plan *p, const double *fbar, const double *u_in, const double *u_out, double a3, double a1,
+                     double b1, double b3, double *ubar, double *acc, hz_stream_t stream)
+{
+    if (!p || !fbar || !u_in || !u_out || !ubar || !acc) return set_error(HZ_ERR_ARG, "null argument");
+    int st;
+    if (!p->d_partial && (st = dmalloc(&p->d_partial, kDotPartials))) return st;
+    return launch_cubic_adjoint(fbar, u_in, u_out, (long long)p->nx * p->ny * p->K, a3, a1, b1, b3, ubar,
+                                p->d_partial, acc, (hipStream_t)stream);
+}
+
 int hz_solve_operators(hz_plan *p, const double *f, const double *gl, const double *gr,
                        const double *op_diag, const double *op_full, double *u, hz_stream_t stream)
 {

@@ -111,6 +111,19 @@ class HelmholtzPlan:
                                       _stream_ptr(stream)))
         return v
 
+    def cubic_adjoint(self, fbar, u_in, u_out, a3, a1, b1, b3, acc, ubar=None, stream=None):
+        """One backward step through solve_cubic's right-hand side
+        (hz_cubic_adjoint): returns ubar = fbar (3 a3 u_in^2 + a1) (in fbar's
+        storage unless ubar is given) and adds sum fbar (b1 (u_out - u_in) +
+        b3 (u_in^3 - u_in)) to the device scalar acc."""
+        for t in (fbar, u_in, u_out):
+            self._check_field(t, (self.nx, self.ny, self.nz))
+        if ubar is None:
+            ubar = fbar
+        check(self.lib.hz_cubic_adjoint(self.handle, _ptr(fbar), _ptr(u_in), _ptr(u_out), float(a3), float(a1),
+                                        float(b1), float(b3), _ptr(ubar), _ptr(acc), _stream_ptr(stream)))
+        return ubar
+
     def solve_operators(self, f, gl, gr, op_diag=None, op_full=None, out=None, stream=None):
         u = out if out is not None else self.empty_field()
         check(self.lib.hz_solve_operators(self.handle, _ptr(f), _ptr(gl), _ptr(gr), _ptr(op_diag),

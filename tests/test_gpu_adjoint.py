@@ -438,6 +438,34 @@ def test_solve_cubic_equals_rhs_then_solve(cuda, shape, form, bc):
     plan.close()
 
 
+def test_cubic_adjoint_matches_torch(cuda):
+    """hz_cubic_adjoint: ubar = fbar (3 a3 u^2 + a1) and the accumulated
+    sum fbar (b1 (u_out - u_in) + b3 (u_in^3 - u_in)) against the same
+    expressions in torch to a few roundings; the sum is the same on every
+    run; in place (ubar = fbar) gives the same bits."""
+    import torch
+    from mpi_amd.solver import HelmholtzPlan
+    plan = HelmholtzPlan(64, 32, 33, 1, lam=2.5e3, alpha_l=1.0, beta_l=0.0, alpha_r=1.0, beta_r=0.0, form="diag")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    shape = (64, 32, plan.nz)
+    fbar, u_in, u_out = (torch.rand(shape, dtype=torch.float64, device="cuda", generator=g) * 2 - 1 for _ in range(3))
+    a3, a1, b1, b3 = 2500.0, -(2500.0 + 2.5e3), -3.1e5, -2.5e5
+    acc = torch.full((), 0.25, dtype=torch.float64, device="cuda")
+    ubar = torch.empty_like(fbar)
+    plan.cubic_adjoint(fbar, u_in, u_out, a3, a1, b1, b3, acc, ubar=ubar)
+    want_u = fbar * (3 * a3 * u_in ** 2 + a1)
+    scale_u = fbar.abs() * (3 * a3 * u_in ** 2 + abs(a1))
+    assert bool(torch.all((ubar - want_u).abs() <= 1e-15 * scale_u + 1e-300))
+    terms = fbar * (b1 * (u_out - u_in) + b3 * (u_in ** 3 - u_in))
+    scale = float(torch.sum((fbar * (abs(b1) * (u_out.abs() + u_in.abs()) + abs(b3) * (u_in.abs() ** 3 + u_in.abs()))).abs()))
+    assert abs(float(acc) - (0.25 + float(torch.sum(terms)))) <= 1e-13 * scale
+    acc2 = torch.full((), 0.25, dtype=torch.float64, device="cuda")
+    f2 = fbar.clone()
+    out = plan.cubic_adjoint(f2, u_in, u_out, a3, a1, b1, b3, acc2)
+    assert out.data_ptr() == f2.data_ptr() and torch.equal(f2, ubar) and float(acc2) == float(acc)
+    plan.close()
+
+
 def test_allen_cahn_graph_refuses_stale_eps(cuda):
     """A captured forward holds its eps in the right-hand side's
     coefficients: after the plan's lambda changes, replay() refuses."""
