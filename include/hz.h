@@ -59,6 +59,11 @@ enum {
  *  DIAG      : z-diagonalised form -- Dzz_int = V diag(mu) V^-1 is computed
  *              once per plan; a solve is two FP64 MFMA contractions per
  *              column batch plus an O(n) shift per column.  No factorisation.
+ *              One element with Dirichlet faces uses the even/odd split of
+ *              the mirror-symmetric GLL operator (hz_host_eigen_split): half
+ *              the contraction work, plus a bf16 first-order correction for
+ *              the operator's rounding-level asymmetry, so the result keeps
+ *              the full basis's accuracy.
  *  MIXED     : FP32 solves through the plan's eigenbasis (two FP32 matrix-core
  *              contractions per column batch) refined with FP64 residuals of
  *              the GLL operator (one FP64 matrix-core contraction per sweep,
