@@ -134,8 +134,8 @@ struct MixedArgs {
     double *spec;               // real plane; imaginary plane at spec + plane_off
     long long plane_off;
     const double *bc_re, *bc_im, *shift;
-    const double *mu;           // [n] eigenvalues of Dzz_int
-    const float *VinvT32, *VT32;  // [NP][NP]: Vinv^T, V^T in FP32 (zero padded)
+    const double *mu;           // [NP] even/odd eigenvalues of the symmetrised Dzz_int (zero padded)
+    const float *VinvT32, *VT32;  // [NP][NP]: Winv^T, (D W)^T of the even/odd basis in FP32
     const double *DzzTP;        // [NP][NP]: DzzTP[k][j] = Dzz[j+1][k+1] (zero padded)
     const double *Dzz, *Dz;     // [N][N] scaled element matrices
     const double *UL, *UR, *hom;
@@ -236,7 +236,7 @@ struct hz_plan {
     double *d_Vrow = nullptr, *d_Vinvrow = nullptr;  // adjoint operands
     // even/odd eigenbasis (one element with Dirichlet faces): mu, w0, wN,
     // r0, rN padded to NP each; operands NP x NP
-    std::vector<double> eo_vec;
+    std::vector<double> eo_vec, eo_W, eo_Wi;  // eo_W, eo_Wi: W, Winv (MIXED's FP32 copies)
     double *d_eo_vec = nullptr, *d_eo_B1 = nullptr, *d_eo_B2 = nullptr, *d_eo_hom = nullptr;
     double *d_eo_B1a = nullptr, *d_eo_B2a = nullptr;  // adjoint operands
     short *d_eo_M = nullptr, *d_eo_Ma = nullptr;      // first-order correction (bf16), forward and adjoint

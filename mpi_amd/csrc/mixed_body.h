@@ -7,14 +7,20 @@
 // (NonPeriodic3D_MPI.cpp:188-201); with Dirichlet data on one element the
 // end values are folded in, b = f_int - Dzz[1..N-2][0] g_l/beta_l
 // - Dzz[1..N-2][N-1] g_r/beta_r, so that x is the whole interior solution.
-// With the plan's eigendecomposition Dzz_int = V diag(mu) V^{-1} held in
-// FP32, one FP32 solve is
-//   x = V32 ((V32^{-1} fl32(b)) / (mu - sigma))        (two FP32 contractions)
+// With the plan's even/odd eigendecomposition of the symmetrised operator,
+// (Dzz_int + J Dzz_int J)/2 = F^{-1} W diag(mu) W^{-1} F (F the fold of
+// mirrored entries, W block diagonal; eigen_decompose_split), held in FP32,
+// one FP32 solve is
+//   x = F^{-1} W32 ((W32^{-1} F fl32(b)) / (mu - sigma))   (two half-size FP32
+//                                                        contractions)
 // and S refinement sweeps follow, each
 //   r = b - Dzz_int x + sigma x                          (FP64 contraction)
-//   x = x + V32 ((V32^{-1} fl32(r)) / (mu - sigma))     (two FP32 contractions)
-// The FP32 solve's backward error does not depend on sigma -- the residual
-// of a solve through the eigenbasis is V d with |d| ~ n eps_32 |V^{-1}| |r|
+//   x = x + F^{-1} W32 ((W32^{-1} F fl32(r)) / (mu - sigma))
+// The residual uses the reference's own operator, so the symmetrisation
+// (a relative change of about 1e-12, the GLL nodes' rounding) is removed by
+// the refinement with the FP32 solve's own error.  The FP32 solve's backward
+// error does not depend on sigma -- the residual of a solve through the
+// eigenbasis is V d with |d| ~ n eps_32 |V^{-1}| |r|
 // -- so every mode contracts by about cond(V) n 2^-24 per sweep (cond(V) is
 // 2.7 at N = 129), the ill-conditioned low-|k| modes included, and the
 // iterate reaches the FP64 solution's accuracy, set by the FP64 residual.
@@ -48,11 +54,11 @@ namespace {
 // column k, row stride LDF), B[k][j] = B[k * NP + j] from L2, loaded two
 // k-steps ahead (as tile_gemm); the k loop stays rolled (the solve runs it
 // in a loop of sweeps, and unrolled it spills the hoisted B loads).
-template <int NP, int LDF, int RBK, int JT>
+template <int NP, int LDF, int RBK, int JT, int KN = NP>
 __device__ __forceinline__ void tile_gemm32(f4 (&acc)[RBK][JT], const float *tile, const float *__restrict__ B,
                                             int c0, int r0, int li, int lk)
 {
-    constexpr int NK = NP / 4;
+    constexpr int NK = KN / 4;
     static_assert(NK % 2 == 0, "two k-steps per iteration");
     const float *bcol = B + 16 * c0 + li + (long long)lk * NP;
     tile += 16 * r0 * LDF;
@@ -111,32 +117,67 @@ struct MixedShared {
 };
 
 // The FP32 solve of the rows' right-hand sides held in sh.t (fl32 of b or of
-// r), added to the iterate (first: stored).  Ends with a barrier.
+// r), added to the iterate (first: stored).  It runs in the even/odd basis
+// of the symmetrised operator (eigen_decompose_split): sh.t is folded in
+// place, each wave contracts over its half's k range (half the matrix work
+// of the full basis), and the result is unfolded into the iterate.  The
+// symmetrised operator differs from the reference's by the rounding of the
+// GLL nodes (about 1e-12 relative), far below the FP32 solve's own error,
+// which the FP64 residuals of the refinement remove.  Ends with a barrier.
 template <int NJ, int RT, bool GENERAL>
 __device__ __forceinline__ void mixed_fp32_solve(const MixedArgs &a, MixedShared<NJ, RT, GENERAL> &sh, bool first,
                                                  int c0, int r0, int li, int lk)
 {
-    constexpr int NP = 16 * NJ;
+    constexpr int NP = 16 * NJ, H = NP / 2;
     constexpr int LD = MixedShared<NJ, RT, GENERAL>::LD, LDF = MixedShared<NJ, RT, GENERAL>::LDF;
     constexpr int WC = NJ % 4 == 0 ? 4 : 2, WR = 4 / WC, JT = NJ / WC, RBW = RT / 16 / WR;
-    const int n = a.n;
+    constexpr int NIT = RT * NP / 256;
+    static_assert(RT * NP % 256 == 0, "whole passes");
+    const int n = a.n, ho = n / 2, he = n - ho;
+    const int kb = 16 * c0 >= H ? H : 0;  // the wave's half
+    const int tid = threadIdx.x;
+    // ---- fold: even coordinate i at i (t_i + t_{n-1-i}, the centre of an odd
+    // n alone), odd coordinate i at H + i (t_i - t_{n-1-i}); zero padding ----
+    {
+        float v[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = tid + 256 * it, row = idx / NP, k = idx % NP;
+            const bool even = k < H;
+            const int i = even ? k : k - H;
+            float x0 = 0.0f, x1 = 0.0f;
+            if (i < (even ? he : ho)) {
+                x0 = sh.t[row * LDF + i];
+                if (i < ho) x1 = sh.t[row * LDF + n - 1 - i];
+            }
+            v[it] = even ? x0 + x1 : x0 - x1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = tid + 256 * it;
+            sh.t[(idx / NP) * LDF + idx % NP] = v[it];
+        }
+        __syncthreads();
+    }
     f4 acc[RBW][JT];
 #pragma unroll
     for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    tile_gemm32<NP, LDF, RBW, JT>(acc, sh.t, a.VinvT32, c0, r0, li, lk);
+    tile_gemm32<NP, LDF, RBW, JT, H>(acc, sh.t + kb, a.VinvT32 + (long long)kb * NP, c0, r0, li, lk);
     __syncthreads();  // every wave has read the right-hand sides
 #pragma unroll
     for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (c0 + jt) + li;
-            const double mu = j < n ? a.mu[j] : 0.0;
+            const bool jv = j < H ? j < he : j - H < ho;
+            const double mu = jv ? a.mu[j] : 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * (r0 + rb) + 4 * lk + r;
-                const float iv = j < n ? (float)(1.0 / (mu - sh.row_sig[row])) : 0.0f;
+                const float iv = jv ? (float)(1.0 / (mu - sh.row_sig[row])) : 0.0f;
                 sh.t[row * LDF + j] = acc[rb][jt][r] * iv;
             }
         }
@@ -145,19 +186,32 @@ __device__ __forceinline__ void mixed_fp32_solve(const MixedArgs &a, MixedShared
     for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[rb][jt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    tile_gemm32<NP, LDF, RBW, JT>(acc, sh.t, a.VT32, c0, r0, li, lk);
+    tile_gemm32<NP, LDF, RBW, JT, H>(acc, sh.t + kb, a.VT32 + (long long)kb * NP, c0, r0, li, lk);
+    __syncthreads();  // every wave has read the scaled coordinates
 #pragma unroll
     for (int rb = 0; rb < RBW; ++rb)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) {
             const int j = 16 * (c0 + jt) + li;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * (r0 + rb) + 4 * lk + r;
-                double &xv = sh.x[row * LD + j];
-                xv = first ? (double)acc[rb][jt][r] : xv + (double)acc[rb][jt][r];
-            }
+            for (int r = 0; r < 4; ++r) sh.t[(16 * (r0 + rb) + 4 * lk + r) * LDF + j] = acc[rb][jt][r];
         }
+    __syncthreads();
+    // ---- unfold into the iterate: x_i += e_i + o_i, x_{n-1-i} += e_i - o_i,
+    // the centre of an odd n e_i; the padding stays zero ----
+    for (int idx = tid; idx < RT * NP; idx += 256) {
+        const int row = idx / NP, k = idx % NP;
+        double &xv = sh.x[row * LD + k];
+        if (k >= n) {
+            if (first) xv = 0.0;
+            continue;
+        }
+        const int i = k < n - 1 - k ? k : n - 1 - k;
+        const float e = sh.t[row * LDF + i];
+        const float o = 2 * k + 1 == n ? 0.0f : sh.t[row * LDF + H + i];
+        const double d = (double)(k <= n - 1 - k ? e + o : e - o);
+        xv = first ? d : xv + d;
+    }
     __syncthreads();
 }
 

@@ -151,9 +151,9 @@ int mixed_operators(hz_plan *p)
     return HZ_OK;
 }
 
-// MIXED form: FP32 copies of the eigenbasis (V^{-T}, V^T) and the padded FP64
-// Dzz_int^T of the residual, all NP x NP; the history and homogeneous
-// solution buffers.
+// MIXED form: FP32 copies of the even/odd eigenbasis (Winv^T, (D W)^T, as
+// the diag kernel's forward operands) and the padded FP64 Dzz_int^T of the
+// residual, all NP x NP; the history and homogeneous solution buffers.
 int setup_mixed(hz_plan *p)
 {
     p->sweeps = p->d.refine_sweeps > 0 ? p->d.refine_sweeps : (p->d.refine_sweeps < 0 ? 0 : 2);
@@ -164,12 +164,15 @@ int setup_mixed(hz_plan *p)
     const int n = p->n, N = p->N, NP = p->NP;
     std::vector<float> VinvT32((size_t)NP * NP, 0.0f), VT32((size_t)NP * NP, 0.0f);
     std::vector<double> DzzTP((size_t)NP * NP, 0.0);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            VinvT32[(size_t)j * NP + i] = (float)p->Vinv[(size_t)i * n + j];
-            VT32[(size_t)j * NP + i] = (float)p->V[(size_t)i * n + j];
-            DzzTP[(size_t)j * NP + i] = p->Dzz[(size_t)(i + 1) * N + (j + 1)];
+    for (int r = 0; r < NP; ++r) {
+        const double dr = (n % 2 && r == n / 2) ? 1.0 : 0.5;
+        for (int c = 0; c < NP; ++c) {
+            VinvT32[(size_t)c * NP + r] = (float)p->eo_Wi[(size_t)r * NP + c];
+            VT32[(size_t)c * NP + r] = (float)(dr * p->eo_W[(size_t)r * NP + c]);
         }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) DzzTP[(size_t)j * NP + i] = p->Dzz[(size_t)(i + 1) * N + (j + 1)];
     int st;
     if ((st = upload(&p->d_VinvT32, VinvT32)) || (st = upload(&p->d_VT32, VT32)) ||
         (st = upload(&p->d_DzzTP, DzzTP)) || (st = dmalloc(&p->d_UL, (size_t)p->modes * n)) ||
@@ -277,6 +280,8 @@ int setup_even_odd(hz_plan *p, const std::vector<double> &A)
         v[3 * NP + j] = (double)t0;
         v[4 * NP + j] = (double)tN;
     }
+    p->eo_W = W;
+    p->eo_Wi = Wi;
     std::vector<double> B1((size_t)NP * NP), B2((size_t)NP * NP), B1a((size_t)NP * NP), B2a((size_t)NP * NP);
     for (int r = 0; r < NP; ++r) {
         const double dr = (n % 2 && r == m) ? 1.0 : 0.5;
@@ -438,7 +443,7 @@ MixedArgs hz::mixed_args(const hz_plan *p)
     a.bc_re = p->bc.re;
     a.bc_im = p->bc.im;
     a.shift = p->d_shift;
-    a.mu = p->d_mu;
+    a.mu = p->d_eo_vec;  // the even/odd eigenvalues, padded to NP
     a.VinvT32 = p->d_VinvT32;
     a.VT32 = p->d_VT32;
     a.DzzTP = p->d_DzzTP;
