@@ -24,20 +24,20 @@ namespace {
 
 // Uniform-base tiles (fft_tile_uniform): one 1-D grid of main-field tiles
 // then extra-field tiles.
-template <int L, int R1, int R2, int W, int KIND, int RB>
+template <int L, int R1, int R2, int W, int KIND, int RB, bool CUB = false>
 __global__ __launch_bounds__((W * fft_threads_per_line<R1, R2, RB>())) void fft_lines(const FftArgs a)
 {
     __shared__ double tile[(L + 2) * W];
-    fft_tile_uniform<L, R1, R2, W, KIND, RB>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
+    fft_tile_uniform<L, R1, R2, W, KIND, RB, CUB>(a, xcd_logical(blockIdx.x, gridDim.x), tile);
 }
 
 // Per-thread addressing, for launches whose offsets do not fit 32 bits.
-template <int L, int R1, int R2, int W, int KIND, int RB>
+template <int L, int R1, int R2, int W, int KIND, int RB, bool CUB = false>
 __global__ __launch_bounds__((W * fft_threads_per_line<R1, R2, RB>())) void fft_lines_wide(const FftArgs a)
 {
     __shared__ double tile[(L + 2) * W];
     const int nbx = gridDim.x, logical = xcd_logical(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
-    fft_tile<L, R1, R2, W, KIND, RB>(a, logical % nbx, logical / nbx, tile);
+    fft_tile<L, R1, R2, W, KIND, RB, CUB>(a, logical % nbx, logical / nbx, tile);
 }
 
 // Largest element index a uniform-base tile of this launch forms, on either
@@ -66,7 +66,12 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
                                 (long long)a.xlanes * ((a.nbatch + W - 1) / W);
         const dim3 grid((unsigned)tiles);
         switch (kind) {
-        case FFT_R2C: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a); break;
+        case FFT_R2C:
+            if (a.pre_cubic)  // the cubic right-hand side formed in the loads (hz_solve_cubic)
+                hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C, RB, true>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a);
+            break;
         case FFT_C2R: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2R, RB>), grid, block, 0, s, a); break;
         case FFT_C2C_FWD: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_FWD, RB>), grid, block, 0, s, a); break;
         case FFT_C2C_INV: hipLaunchKernelGGL((fft_lines<L, R1, R2, W, FFT_C2C_INV, RB>), grid, block, 0, s, a); break;
@@ -74,7 +79,12 @@ int launch_kind(FftKind kind, const FftArgs &a, hipStream_t s)
     } else {
         const dim3 grid((a.lanes + a.xlanes + W - 1) / W, a.nbatch);
         switch (kind) {
-        case FFT_R2C: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a); break;
+        case FFT_R2C:
+            if (a.pre_cubic)
+                hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C, RB, true>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_R2C, RB>), grid, block, 0, s, a);
+            break;
         case FFT_C2R: hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2R, RB>), grid, block, 0, s, a); break;
         case FFT_C2C_FWD:
             hipLaunchKernelGGL((fft_lines_wide<L, R1, R2, W, FFT_C2C_FWD, RB>), grid, block, 0, s, a);

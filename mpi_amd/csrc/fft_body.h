@@ -94,30 +94,38 @@ __device__ __forceinline__ void dft_regs(double (&re)[R], double (&im)[R], const
 
 // Addressing of a tile's operands.  Point p of the thread's line is at
 // element index off(p) = base_offset + p * point_stride of each operand.
-// Addr64: per-thread pointers and 64-bit indices (any lane mix, any size).
+// Addr64T: per-thread pointers and 64-bit indices (any lane mix, any size).
 // A lane with `cub` set reads x * (c3 x^2 + c1) for the real input x (the
 // cubic right-hand side hz_solve_cubic forms inside the R2C pass; the same
-// operations as hz_cubic_rhs).
+// operations as hz_cubic_rhs).  Only the CUB instances of the tile functions
+// carry the map, so the plain transforms keep their loads as they are.
 __device__ __forceinline__ double cubic_map(double x, bool cub, double c3, double c1)
 {
     return cub ? x * fma(c3 * x, x, c1) : x;
 }
 
-struct Addr64 {
+template <bool CUB>
+struct Addr64T {
     gcdouble *ire, *iim;
     gdouble *ore, *oim;
     long long ib, ips, ob, ops;
     bool cub;
     double c3, c1;
-    __device__ __forceinline__ double ldr(int p) const { return cubic_map(ire[ib + (long long)p * ips], cub, c3, c1); }
+    __device__ __forceinline__ double ldr(int p) const
+    {
+        const double v = ire[ib + (long long)p * ips];
+        if constexpr (CUB) return cubic_map(v, cub, c3, c1);
+        return v;
+    }
     __device__ __forceinline__ double ldi(int p) const { return iim[ib + (long long)p * ips]; }
     __device__ __forceinline__ void str(int p, double v) const { ore[ob + (long long)p * ops] = v; }
     __device__ __forceinline__ void sti(int p, double v) const { oim[ob + (long long)p * ops] = v; }
 };
-// Addr32: wave-uniform base pointers and 32-bit per-thread byte offsets (the
+// Addr32T: wave-uniform base pointers and 32-bit per-thread byte offsets (the
 // host guarantees every offset of the launch is below 2^32 bytes), so each
 // access is a scalar base plus one offset register.
-struct Addr32 {
+template <bool CUB>
+struct Addr32T {
     gcdouble *ire, *iim;
     gdouble *ore, *oim;
     unsigned ib, ips, ob, ops;
@@ -129,7 +137,12 @@ struct Addr32 {
         typedef __attribute__((address_space(1))) char gchar;
         return (P)((gchar *)base + 8u * idx);
     }
-    __device__ __forceinline__ double ldr(int p) const { return cubic_map(*at(ire, ib + (unsigned)p * ips), cub, c3, c1); }
+    __device__ __forceinline__ double ldr(int p) const
+    {
+        const double v = *at(ire, ib + (unsigned)p * ips);
+        if constexpr (CUB) return cubic_map(v, cub, c3, c1);
+        return v;
+    }
     __device__ __forceinline__ double ldi(int p) const { return *at(iim, ib + (unsigned)p * ips); }
     __device__ __forceinline__ void str(int p, double v) const { *at(ore, ob + (unsigned)p * ops) = v; }
     __device__ __forceinline__ void sti(int p, double v) const { *at(oim, ob + (unsigned)p * ops) = v; }
@@ -488,7 +501,7 @@ __device__ __forceinline__ void fft_core_any(const A &ad, bool live, const doubl
 // One (lane block, line batch) tile, lanes = the main field's z lanes then
 // the extra single-lane fields (faces), per-thread addressing (the batch
 // kernels' tile).
-template <int L, int R1, int R2, int W, int KIND, int RB = 1>
+template <int L, int R1, int R2, int W, int KIND, int RB = 1, bool CUB = false>
 __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int batch, double *tile)
 {
     const int w = threadIdx.x % W;
@@ -498,7 +511,7 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
     // of the arguments that every data load then waits behind).
     const bool xl = lane >= a.lanes && lane < a.lanes + a.xlanes;
     const bool x1 = lane - a.lanes == 1;
-    Addr64 ad;
+    Addr64T<CUB> ad;
     ad.ire = (gcdouble *)(xl ? (x1 ? sval(a.xin_re[1]) : sval(a.xin_re[0])) : sval(a.in_re));
     ad.iim = (gcdouble *)(xl ? (x1 ? sval(a.xin_im[1]) : sval(a.xin_im[0])) : sval(a.in_im));
     ad.ore = (gdouble *)(xl ? (x1 ? sval(a.xout_re[1]) : sval(a.xout_re[0])) : sval(a.out_re));
@@ -507,7 +520,7 @@ __device__ __forceinline__ void fft_tile(const FftArgs &a, int lane_block, int b
     ad.ips = xl ? sval(a.x_in_point_stride) : sval(a.in_point_stride);
     ad.ob = xl ? (long long)batch * sval(a.x_out_batch_stride) : (long long)batch * sval(a.out_batch_stride) + lane;
     ad.ops = xl ? sval(a.x_out_point_stride) : sval(a.out_point_stride);
-    ad.cub = KIND == FFT_R2C && a.pre_cubic && !xl;
+    ad.cub = !xl;  // the faces take no map
     ad.c3 = a.pre_c3;
     ad.c1 = a.pre_c1;
     fft_core_any<L, R1, R2, RB, W, KIND>(ad, lane < a.lanes || xl, a.tw, tile);
@@ -537,7 +550,7 @@ __host__ __device__ inline long long uniform_main_tiles(int nbatch, int lanes, l
 // when lanes_merge holds); the rest are the extra fields,
 // xlanes * ceil(nbatch / W) tiles whose W "lanes" are W consecutive line
 // batches of one extra field.
-template <int L, int R1, int R2, int W, int KIND, int RB = 1>
+template <int L, int R1, int R2, int W, int KIND, int RB = 1, bool CUB = false>
 __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double *tile)
 {
     const int w = threadIdx.x % W;
@@ -547,7 +560,7 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
     const int lanes = merged ? nb * ln : ln;
     const int nlb = (lanes + W - 1) / W;
     const int nmain = (merged ? 1 : nb) * nlb;
-    Addr32 ad;
+    Addr32T<CUB> ad;
     bool live;
     if (t < nmain) {
         const int batch = t / nlb, l0 = (t % nlb) * W;
@@ -561,7 +574,7 @@ __device__ __forceinline__ void fft_tile_uniform(const FftArgs &a, int t, double
         ad.ob = w;
         ad.ips = (unsigned)a.in_point_stride;
         ad.ops = (unsigned)a.out_point_stride;
-        ad.cub = KIND == FFT_R2C && a.pre_cubic;
+        ad.cub = true;
         live = l0 + w < lanes;
     } else {
         const int nbb = (nb + W - 1) / W;
