@@ -98,9 +98,11 @@ def whole_solve_roofline(form, ms_per_solve, nx, ny, n_gll, refine_sweeps=2, cei
         mode_flop, mode_bytes = lu_flops(nx, ny, n_gll), 2 * half
     elif form == "diag":
         mode_flop, mode_bytes = diag_flops(nx, ny, n_gll), 2 * half
-    else:  # FP32 eigenbasis solve (two contractions) + per sweep one FP64 residual contraction and
-        # another FP32 solve; the FP32 contractions priced at half the FP64 time (2x the matrix rate)
-        mode_flop = cols * (2 * n * n * refine_sweeps + 0.5 * 4 * n * n * (refine_sweeps + 1))
+    else:  # FP32 even/odd eigenbasis solve (two half-size contractions) + per sweep one FP64 residual
+        # contraction and another FP32 solve; the FP32 contractions priced at half the FP64 time
+        ho = n // 2
+        he = n - ho
+        mode_flop = cols * (2 * n * n * refine_sweeps + 0.5 * 4 * (he * he + ho * ho) * (refine_sweeps + 1))
         mode_bytes = 2 * half
     phases = {"fft_forward": (0.0, real + half), "mode_solve": (mode_flop, mode_bytes),
               "fft_inverse": (0.0, half + real)}
@@ -409,7 +411,11 @@ def time_precision(steps=5):
     by = {r["sweeps"]: r["ms_per_solve"] for r in per_sweep}
     if 1 in by and 2 in by:
         dt = (by[2] - by[1]) * 1e-3
-        f64, f32 = 2.0 * n * n * cols, 4.0 * n * n * cols
+        # the FP32 solve runs in the even/odd basis: two contractions of
+        # ceil(n/2)^2 + floor(n/2)^2 multiply-adds per column
+        ho = n // 2
+        he = n - ho
+        f64, f32 = 2.0 * n * n * cols, 4.0 * (he * he + ho * ho) * cols
         res["refinement_sweep"] = {
             "ms": dt * 1e3, "fp64_residual_flop": f64, "fp32_solve_flop": f32,
             "ideal_ms": (f64 / PEAK_FP64 + f32 / 157.3e12) * 1e3,
@@ -543,7 +549,9 @@ def main():
         # the stage's event pair inside a solve also holds the idle gap an
         # event pair between two kernels costs
         tun = chosen.get("tuning") or {}
-        tk = (tun.get("choices") or {}).get(str(tun.get("diag_tile_rows")))
+        ch = tun.get("choices") or {}
+        rt = tun.get("diag_tile_rows")
+        tk = ch.get(rt, ch.get(str(rt)))  # keyed by the tile height
         stage_event_ms, ms = ms, (tk if tk else ms)
         work = diag_flops(NX, NY, N)
         dense = diag_flops(NX, NY, N, even_odd=False)
