@@ -149,9 +149,12 @@ struct Addr32T {
 };
 
 // The transform of one tile: W lines (thread lane w), L points each, W *
-// max(R1, R2) threads; `tile` holds (L + 2) * W doubles of LDS (the C2R
-// half spectrum; then the exchange, real parts, then imaginary parts).  Starts with a barrier so that back-to-back
-// tiles can reuse the LDS.
+// max(R1, R2) threads; `tile` holds (L + 2) * W doubles of LDS (the exchange,
+// real parts, then imaginary parts).  C2R reads the half spectrum, point p
+// above L/2 as the conjugate of L - p, straight from memory (the second
+// read of a point comes from the caches; staging the half spectrum in LDS
+// instead cost a barrier and an LDS round trip, ~1 us of a 256^2 x 129
+// pass).  Starts with a barrier so that back-to-back tiles can reuse the LDS.
 template <int L, int R1, int R2, int W, int KIND, class A>
 __device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *__restrict__ tw, double *tile)
 {
@@ -162,42 +165,21 @@ __device__ __forceinline__ void fft_core(const A &ad, bool live, const double2 *
 
     double sre[R1], sim[R1];
     if constexpr (KIND == FFT_C2R) {
-        // The half spectrum (points 0..L/2) is read once into the tile
-        // (real parts, then imaginary parts), and the mirrored points
-        // L/2+1..L-1 are taken from there conjugated: each point is loaded
-        // from memory once instead of twice.
-        constexpr int H = L / 2 + 1, Q = R1 > R2 ? R1 : R2;
-        constexpr int NJ = (H + Q - 1) / Q;
-        double hr[NJ], hi[NJ];
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const int pt = q + Q * j;
-            hr[j] = hi[j] = 0.0;
-            if (live && pt < H) {
-                hr[j] = ad.ldr(pt);
-                hi[j] = ad.ldi(pt);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const int pt = q + Q * j;
-            if (pt < H) {
-                tile[pt * W + w] = hr[j];
-                tile[(H + pt) * W + w] = hi[j];
-            }
-        }
-        __syncthreads();
+        // mirrored points loaded again (from the caches) instead of staged
         if (q < R2) {
 #pragma unroll
             for (int n1 = 0; n1 < R1; ++n1) {
                 const int p = q + R2 * n1;
                 const int pp = p <= L / 2 ? p : L - p;
-                sre[n1] = tile[pp * W + w];
-                const double vi = tile[(H + pp) * W + w];
+                double vr = 0.0, vi = 0.0;
+                if (live) {
+                    vr = ad.ldr(pp);
+                    vi = ad.ldi(pp);
+                }
+                sre[n1] = vr;
                 sim[n1] = p > L / 2 ? -vi : vi;
             }
         }
-        __syncthreads();  // every staged point is read before the tile is reused
     } else if (q < R2) {
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
