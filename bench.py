@@ -181,13 +181,11 @@ def reference_sample_error(u_host):
     return float(np.abs(got - want).max() / rec["max_abs_u"])
 
 
-def time_form(form, f, gl, gr, steps, warmup):
+def time_form(plan, form, df, dgl, dgr, steps, warmup):
+    """Times `steps` solves of an already set-up plan on resident inputs;
+    returns the record and the solution (checked against the reference's
+    sample after every form has been timed)."""
     import torch
-    from mpi_amd.solver import HelmholtzPlan
-    plan = HelmholtzPlan(NX, NY, N, 1, lam=LAM, form=form)
-    df = torch.from_numpy(f).cuda()
-    dgl = torch.from_numpy(gl).cuda()
-    dgr = torch.from_numpy(gr).cuda()
     u = plan.empty_field()
     for _ in range(warmup):
         plan.solve(df, dgl, dgr, out=u)
@@ -207,21 +205,19 @@ def time_form(form, f, gl, gr, steps, warmup):
     stages, nrec = plan.profile_read()
     plan.profile(False)
     plan.status()
-    err = reference_sample_error(u.cpu().numpy())
     res = {"form": form, "solves_per_s": steps / (t1 - t0), "ms_per_solve": 1e3 * (t1 - t0) / steps,
-           "stage_ms": stages, "rel_err_vs_reference": err, "profiled_solves": nrec, "tuning": plan.tuning()}
+           "stage_ms": stages, "profiled_solves": nrec, "tuning": plan.tuning()}
     plan.close()
-    return res
+    return res, u
 
 
-def time_adjoint(f, gl, gr, steps, warmup, fwd_ms):
-    """Adjoint of the chosen (DIAG) solve at the same size: dL/df, dL/dg from
-    a seeded dL/du, timed like the forward; then once more with dL/dlambda
-    (its scalar comes back to the host, so that call synchronises)."""
+def time_adjoint(plan, df, dgl, dgr, steps, warmup, fwd_ms):
+    """Adjoint of the chosen (DIAG) solve at the same size on an already
+    set-up plan: dL/df, dL/dg from a seeded dL/du, timed like the forward;
+    then once more with dL/dlambda (its scalar comes back to the host, so
+    that call synchronises)."""
     import torch
-    from mpi_amd.solver import HelmholtzPlan
-    plan = HelmholtzPlan(NX, NY, N, 1, lam=LAM, form="diag")
-    u = plan.solve(torch.from_numpy(f).cuda(), torch.from_numpy(gl).cuda(), torch.from_numpy(gr).cuda())
+    u = plan.solve(df, dgl, dgr)
     g = torch.Generator(device="cuda").manual_seed(3)
     ubar = torch.randn(u.shape, dtype=torch.float64, device="cuda", generator=g)
     for _ in range(warmup):
@@ -528,10 +524,26 @@ def main():
     f, gl, gr = make_inputs(probe)
     probe.close()
 
-    results = {}
-    for form in args.forms.split(","):
-        results[form] = time_form(form, f, gl, gr, args.steps, args.warmup)
+    # every plan is set up (host eigendecomposition, tile tuning) before the
+    # first timed window, so that no window follows a stretch of idle GPU:
+    # 20 solves timed right after a plan's set-up ran at 0.178-0.185 ms per
+    # solve at 256^2 x 129, the same solves at 0.163 ms once the GPU had been
+    # busy for a few milliseconds (its clocks rise under sustained load)
+    forms = args.forms.split(",")
+    plans = {form: HelmholtzPlan(NX, NY, N, 1, lam=LAM, form=form) for form in forms}
+    extras = not args.no_extras and "diag" in forms
+    adj_plan = HelmholtzPlan(NX, NY, N, 1, lam=LAM, form="diag") if extras else None
+    df, dgl, dgr = (torch.from_numpy(x).cuda() for x in (f, gl, gr))
+    results, sols = {}, {}
+    for form in forms:
+        results[form], sols[form] = time_form(plans.pop(form), form, df, dgl, dgr, args.steps, args.warmup)
         progress(f"{form}: {results[form]['ms_per_solve']:.4f} ms per solve")
+    if extras:
+        adjoint = time_adjoint(adj_plan, df, dgl, dgr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
+        progress("adjoint done")
+    for form in forms:
+        results[form]["rel_err_vs_reference"] = reference_sample_error(sols.pop(form).cpu().numpy())
+    del df, dgl, dgr
     ok = [r for r in results.values() if r["rel_err_vs_reference"] <= 1e-10]
     chosen = max(ok, key=lambda r: r["solves_per_s"]) if ok else results["plain"]
 
@@ -598,9 +610,8 @@ def main():
                     "solve (one eigendecomposition per plan), so this rate can exceed the FP64 peak and is not a "
                     "faster LU"},
     }
-    if not args.no_extras and "diag" in results:
-        line["adjoint"] = time_adjoint(f, gl, gr, args.steps, args.warmup, results["diag"]["ms_per_solve"])
-        progress("adjoint done")
+    if extras:
+        line["adjoint"] = adjoint
         line["allen_cahn"] = time_allen_cahn()
         progress("allen_cahn done")
         line["batch"] = time_batch(ceilings=roof["attainable"])
