@@ -137,3 +137,30 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+@pytest.mark.parametrize("change,code", [
+    ({"nx": 96}, "HZ_ERR_NOT_POW2"), ({"ny": 3}, "HZ_ERR_NOT_POW2"), ({"nx": 0}, "HZ_ERR_NOT_POW2"),
+    ({"n": 2}, "HZ_ERR_N_TOO_SMALL"), ({"nx": 2048}, "HZ_ERR_UNSUPPORTED"), ({"ne": 0}, "HZ_ERR_ARG"),
+    ({"zr": -1.0}, "HZ_ERR_ARG"), ({"xr": 0.0}, "HZ_ERR_ARG"), ({"form": 3}, "HZ_ERR_ARG"),
+    ({"form": -1}, "HZ_ERR_ARG"), ({}, "HZ_ERR_NO_DEVICE")])
+def test_plan_create_refusals_before_the_device(change, code):
+    """hz_plan_create checks the descriptor before it looks for a device: the
+    reference's own refusals (non-power-of-two Fourier sizes,
+    Fourier_Quadrature.cpp:1106-1114; N < 3, GLLQuadrature.cpp:82-83) and the
+    argument errors come back as their codes, a valid descriptor on a host
+    without a GPU as HZ_ERR_NO_DEVICE, and no plan is returned."""
+    import torch
+    from mpi_amd import _lib
+    if not change and torch.cuda.device_count() > 0:
+        pytest.skip("a HIP device is present: the valid descriptor creates a plan")
+    lib = _lib.load()
+    fields = dict(nx=64, ny=32, n=33, ne=1, xl=0.0, xr=2 * math.pi, yl=0.0, yr=2 * math.pi, zl=-1.0, zr=1.0,
+                  lam=1.0, alpha_l=0.0, beta_l=1.0, alpha_r=0.0, beta_r=1.0, form=1, refine_sweeps=2,
+                  mixed_fp64_cond=0.0, periodic_z=0)
+    fields.update(change)
+    desc = _lib.PlanDesc(**fields)
+    handle = ctypes.c_void_p(12345)
+    rc = lib.hz_plan_create(ctypes.byref(desc), ctypes.byref(handle))
+    assert rc == getattr(_lib, code)
+    assert handle.value is None

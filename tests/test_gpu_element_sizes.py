@@ -1,7 +1,10 @@
-"""The smallest elements the plan accepts: N = 3 (one interior node, so the
-even/odd split has an empty odd half) and N = 4 (one node in each half), for
-every form, Dirichlet and Robin faces, one to three elements, against the
-oracle; and the adjoint against the oracle's transposed solve map.
+"""Element sizes at both ends of what a plan accepts.  The smallest, N = 3
+(one interior node, so the even/odd split has an empty odd half) and N = 4
+(one node in each half), for every form, Dirichlet and Robin faces, one to
+three elements, against the oracle; and the adjoint against the oracle's
+transposed solve map.  The largest each form takes (N - 2 = 128 for PLAIN,
+256 for DIAG and MIXED) against the oracle, and one more node refused at plan
+creation with HZ_ERR_UNSUPPORTED.
 
 Bar as in the DIAG tests: relative max-norm error <= max(1e-12, 2 x the
 rounding spread of the reference algorithm on the same input: the oracle built
@@ -88,3 +91,32 @@ def test_smallest_elements_adjoint(cuda, form, bc, N):
     got = np.concatenate([fbar.cpu().numpy().ravel(), glbar.cpu().numpy().ravel(), grbar.cpu().numpy().ravel()])
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
     plan.close()
+
+
+@pytest.mark.parametrize("form,N", [("plain", 130), ("diag", 258), ("mixed", 258)])
+@pytest.mark.parametrize("bc", [DIRICHLET, ROBIN], ids=["dirichlet", "robin"])
+def test_largest_elements_against_oracle(cuda, form, N, bc):
+    nx, ny, lam = 4, 8, 3.0
+    plan = make_plan(nx, ny, N, 1, bc, lam, form)
+    desc = make_desc(nx, ny, N, 1, bc, lam)
+    rng = np.random.default_rng(N + (bc == ROBIN))
+    f = rng.standard_normal((nx, ny, plan.nz))
+    gl = rng.standard_normal((nx, ny))
+    gr = rng.standard_normal((nx, ny))
+    print(f"case form={form} bc={'robin' if bc == ROBIN else 'dirichlet'} N={N}", flush=True)
+    u = plan.solve(dev(f), dev(gl), dev(gr)).cpu().numpy()
+    plan.status()
+    uo = po.solve(desc, f, gl, gr)
+    bar = max(TOL, 2.0 * rel_err(po.solve(desc, f, gl, gr, fma=True), uo))
+    err = rel_err(u, uo)
+    print(f"rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
+    plan.close()
+
+
+@pytest.mark.parametrize("form,N", [("plain", 131), ("diag", 259), ("mixed", 259)])
+def test_elements_past_the_limit_refused(cuda, form, N):
+    from mpi_amd import _lib
+    with pytest.raises(_lib.HzError) as e:
+        make_plan(4, 8, N, 1, DIRICHLET, 3.0, form)
+    assert e.value.code == _lib.HZ_ERR_UNSUPPORTED
