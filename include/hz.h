@@ -292,6 +292,15 @@ int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *diag_ms, in
  * rows).  For tests and tuning studies; no counterpart in the reference. */
 int hz_plan_diag_candidates(const hz_plan *plan, int *tile_rows, int max, int *count);
 int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
+/* Runs the plan-time autotuning again (transform lane widths; DIAG tile
+ * rows), e.g. once the GPU has reached its clocks under load: the plan's
+ * device must be current.  Synchronises the device first (the tuners time on
+ * the plan's own spectrum and face buffers, which a solve in flight would
+ * share), allocates a scratch field for the transform timings, and leaves
+ * the new choices and times for hz_plan_tuning.  Results of later solves are
+ * unchanged: every choice gives the same output bit for bit.  No counterpart
+ * in the reference. */
+int hz_plan_retune(hz_plan *plan);
 /* Periodic-z plans: the closure coefficient of the cyclic element-interface
  * system per real spectral column of the last solve (the reference's
  * alphaPDE, Periodic3D_MPI.cpp:698), half-spectrum order:

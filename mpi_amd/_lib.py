@@ -72,6 +72,7 @@ SIGNATURES = {
     "hz_plan_tuning": (ctypes.c_int, [_VP, _PI, _PD, ctypes.c_int, _PI, _PD]),
     "hz_plan_diag_candidates": (ctypes.c_int, [_VP, _PI, ctypes.c_int, _PI]),
     "hz_plan_set_diag_tile": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "hz_plan_retune": (ctypes.c_int, [_VP]),
     "hz_host_plan_operators": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD, _PD]),
     "hz_host_eigen": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _PD, _PD, _PD]),
     "hz_host_eigen_split": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _PD, _PD, _PD, _PD]),

@@ -664,21 +664,30 @@ int tune_diag(hz_plan *p)
     HZ_HIP(hipEventCreate(&e1));
     double best = 1e300;
     int st = HZ_OK;
+    std::vector<double> ms_of(k, 1e300);
+    // rounds over the choices in turn (so that clock drift hits all of them
+    // alike), each choice's best round of 3 back-to-back launches kept
+    for (int round = 0; round < 3 && !st; ++round) {
+        for (int i = 0; i < k && !st; ++i) {
+            if (choices[i] < 2 * p->ne) continue;  // a tile must hold one column's elements
+            DiagArgs a = diag_args(p);
+            a.rt = choices[i];
+            if (round == 0 && (st = launch_diag_modes(a, s))) break;  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && !st; ++r) st = launch_diag_modes(a, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms_of[i] = std::min(ms_of[i], ms / 3.0);
+        }
+    }
     p->diag_rt_ms.assign(k, 0.0);
     for (int i = 0; i < k && !st; ++i) {
-        if (choices[i] < 2 * p->ne) continue;  // a tile must hold one column's elements
-        DiagArgs a = diag_args(p);
-        a.rt = choices[i];
-        if ((st = launch_diag_modes(a, s))) break;
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < 3 && !st; ++r) st = launch_diag_modes(a, s);
-        (void)hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        p->diag_rt_ms[i] = ms / 3.0;
-        if (ms < best) {
-            best = ms;
+        if (ms_of[i] == 1e300) continue;
+        p->diag_rt_ms[i] = ms_of[i];
+        if (ms_of[i] < best) {
+            best = ms_of[i];
             p->diag_rt = choices[i];
         }
     }
@@ -1195,6 +1204,16 @@ int hz_plan_set_diag_tile(hz_plan *p, int tile_rows)
     if (tile_rows < 2 * p->ne) return set_error(HZ_ERR_ARG, "the tile does not hold one column's elements");
     p->diag_rt = tile_rows;
     return HZ_OK;
+}
+
+int hz_plan_retune(hz_plan *p)
+{
+    if (!p) return set_error(HZ_ERR_ARG, "null argument");
+    // the tuners time on the plan's own spectrum and face buffers
+    HZ_HIP(hipDeviceSynchronize());
+    int st = tune_fft(p);
+    if (!st && p->d.form == HZ_FORM_DIAG) st = tune_diag(p);
+    return st;
 }
 
 int hz_plan_periodic_closure(const hz_plan *p, double *alpha)

@@ -228,6 +228,11 @@ class HelmholtzPlan:
         """Pins the DIAG mode kernel's shape to one of diag_candidates()."""
         check(self.lib.hz_plan_set_diag_tile(self.handle, int(rows)))
 
+    def retune(self):
+        """Runs the plan's autotuning again (synchronises the device); the
+        new choices and times show in tuning()."""
+        check(self.lib.hz_plan_retune(self.handle))
+
     def periodic_closure(self):
         """Closure coefficients of the last periodic-z solve, [2][nx][ny//2+1]
         (real, imaginary plane); zeros with one element."""

@@ -125,6 +125,34 @@ def test_diag_plan_autotune(cuda, N):
     err, bar = rel_err(u, uo), spread_bar(desc, f, g, g, uo)
     print(f"autotuned N={N}: rel err {err:.2e}, bar {bar:.2e}")
     assert err <= bar
+    # tuning again (on the plan's own buffers) leaves valid choices and the
+    # same solve bit for bit
+    plan.retune()
+    t2 = plan.tuning()
+    print(f"retuned N={N}: {t2}")
+    assert t2["diag_tile_rows"] in set(plan.diag_candidates()) and len(t2["choice_ms"]) == len(t["choice_ms"])
+    assert t2["fft_lanes"][0] == 0 and t2["fft_lanes"][1] in {8, 16, 32}
+    u2 = plan.solve(to_dev(f), to_dev(g), to_dev(g)).cpu().numpy()
+    assert np.array_equal(u2, u)
+    plan.close()
+
+
+@pytest.mark.parametrize("form", ["plain", "mixed"])
+def test_retune_other_forms(cuda, form):
+    """hz_plan_retune on the forms without a tile tuner re-times the
+    transform lane widths only; the solve stays the same bit for bit."""
+    from mpi_amd.solver import HelmholtzPlan
+    plan = HelmholtzPlan(64, 32, 33, 1, lam=2.0, form=form)
+    rng = np.random.default_rng(5)
+    f = to_dev(rng.standard_normal((64, 32, plan.nz)))
+    g = to_dev(rng.standard_normal((64, 32)))
+    u = plan.solve(f, g, g).cpu().numpy()
+    plan.retune()
+    t = plan.tuning()
+    # nx = 64: three lane widths for the x passes; ny = 32: one shape (0) for the y passes
+    assert t["diag_tile_rows"] == 0 and t["fft_lanes"][0] == 0 and t["fft_lanes"][3] == 0
+    assert t["fft_lanes"][1] in {8, 16, 32} and t["fft_lanes"][2] in {8, 16, 32}
+    assert np.array_equal(plan.solve(f, g, g).cpu().numpy(), u)
     plan.close()
 
 
