@@ -51,6 +51,18 @@ def test_dropin_diag_form(cuda):
     assert err <= 2.0e-13
 
 
+def test_dropin_beyond_the_reference_kernel(cuda):
+    """One element of N = 258 with the default SolverForm: more than the
+    reference-algorithm kernel holds (N - 2 <= 128), so CreatePDEPlan makes a
+    z-diagonalised plan and the standard operator runs on it.  The reference
+    algorithm's own error against the exact solution at this size is 2.3e-11
+    (the oracle on the same problem)."""
+    r = run("large")
+    assert r.returncode == 0, r.stdout + r.stderr
+    err, _ = errors(r.stdout)
+    assert err <= 1e-10, err
+
+
 def test_singular_operator_exits_like_reference(cuda):
     r = run("singular")
     assert r.returncode == 1
