@@ -298,8 +298,10 @@ int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
  * the plan's own spectrum and face buffers, which a solve in flight would
  * share), allocates a scratch field for the transform timings, and leaves
  * the new choices and times for hz_plan_tuning.  Results of later solves are
- * unchanged: every choice gives the same output bit for bit.  No counterpart
- * in the reference. */
+ * unchanged: every DIAG tile height gives the same output bit for bit, and a
+ * transform's lane width sets how many lines one work-group takes, not any
+ * line's arithmetic (the radix split depends on the length only).  No
+ * counterpart in the reference. */
 int hz_plan_retune(hz_plan *plan);
 /* Periodic-z plans: the closure coefficient of the cyclic element-interface
  * system per real spectral column of the last solve (the reference's
