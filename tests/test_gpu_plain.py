@@ -36,9 +36,12 @@ def make_plan(desc_dict, form="plain", size=1):
 
 
 @pytest.mark.parametrize("nx,ny,lanes", [(32, 32, 33), (8, 4, 3), (256, 256, 129), (16, 64, 5), (2, 2, 1),
-                                         (512, 512, 7), (1024, 8, 2)])
+                                         (512, 512, 7), (1024, 8, 2), (128, 128, 65), (128, 1024, 17),
+                                         (1024, 128, 1), (1, 64, 4), (64, 1, 4), (1, 1, 9)])
 def test_fourier_xy_matches_dft(cuda, nx, ny, lanes):
-    """Forward half spectrum equals numpy's rfft2 over (x, y); inverse round-trips."""
+    """Forward half spectrum equals numpy's rfft2 over (x, y); inverse round-trips.
+    Every length from 1 to 1024, and lane counts that leave a one-lane
+    remainder group (17, 33, 65, 129) or a single lane."""
     from mpi_amd.solver import HelmholtzPlan
     plan = HelmholtzPlan(nx, ny, 5, 1)
     rng = np.random.default_rng(nx * 7 + ny)
