@@ -134,3 +134,44 @@ def test_lowdim_limits(cuda):
         HelmholtzPlan(8, 8, 8, 64, form="plain")      # 2 + 4*64 > 256 with nx >= 4
     p = HelmholtzPlan(1, 1, 8, 127, form="plain")     # the largest PLAIN 1-D plan
     p.close()
+
+
+@pytest.mark.parametrize("form", ["diag", "mixed", "plain"])
+@pytest.mark.parametrize("N", [17, 129])
+def test_lowdim_one_dirichlet_element(cuda, form, N):
+    """One element with Dirichlet faces (the even/odd split of the DIAG and
+    MIXED forms, forward and adjoint) in a 2-D plan, both mappings, and a 1-D
+    plan: against the oracle (x a single point) within max(1e-12, 2 x its
+    rounding spread), and <S x, ubar> = <x, S^T ubar> for the adjoint."""
+    import torch
+    from mpi_amd.solver import HelmholtzPlan
+    rng = np.random.default_rng(N + len(form))
+    lam, K = 2.5, N
+    for lines, mapping in ((64, "y"), (64, "x"), (1, "1")):
+        print(f"case form={form} N={N} mapping={mapping}", flush=True)
+        f = rng.standard_normal((lines, K))
+        gl = rng.standard_normal(lines)
+        gr = rng.standard_normal(lines)
+        d = po.make_desc(1, lines, N, N, size=1, xl=0.0, xr=1.0, yl=0.0, yr=3.0, zl=-1.0, zr=1.0, lam=lam)
+        args = (f.reshape(1, lines, K).copy(), gl.reshape(1, lines).copy(), gr.reshape(1, lines).copy())
+        uo = po.solve(d, *args).reshape(lines, K)
+        uf = po.solve(d, *args, fma=True).reshape(lines, K)
+        bar = max(1e-12, 2.0 * float(np.abs(uf - uo).max() / np.abs(uo).max()))
+        if mapping == "x":
+            shape = (lines, 1)
+            plan = HelmholtzPlan(lines, 1, N, 1, xl=0.0, xr=3.0, yl=0.0, yr=1.0, zl=-1.0, zr=1.0, lam=lam, form=form)
+        else:
+            shape = (1, lines)
+            plan = HelmholtzPlan(1, lines, N, 1, xl=0.0, xr=1.0, yl=0.0, yr=3.0, zl=-1.0, zr=1.0, lam=lam, form=form)
+        F, GL, GR = dev(f.reshape(shape + (K,))), dev(gl.reshape(shape)), dev(gr.reshape(shape))
+        u = plan.solve(F, GL, GR)
+        plan.status()
+        err = float(np.abs(u.cpu().numpy().reshape(lines, K) - uo).max() / np.abs(uo).max())
+        print(f"  rel err {err:.2e}, bar {bar:.2e}")
+        assert err <= bar
+        ubar = torch.from_numpy(rng.standard_normal(shape + (K,))).cuda()
+        fbar, glbar, grbar, _ = plan.adjoint(ubar)
+        lhs = float((u * ubar).sum())
+        rhs = float((F * fbar).sum() + (GL * glbar).sum() + (GR * grbar).sum())
+        plan.close()
+        assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs)), (mapping, lhs, rhs)
