@@ -278,30 +278,36 @@ int hz_solve_batched(hz_plan *const *plans, int count, const double *const *f, c
  * (S, at most max_sweeps).  No counterpart in the reference (its LU is FP64
  * only). */
 int hz_mixed_history(const hz_plan *plan, double *rel_residual, int max_sweeps, int *sweeps);
-/* Plan-time autotuning results: tile rows of the diagonalised mode kernel
- * (DIAG plans; 0 otherwise) with the measured ms of each candidate (0 for
- * a candidate that does not apply), and the
+/* Plan-time autotuning results: the mode kernel's shape -- tile rows of the
+ * diagonalised kernel (DIAG plans), the column capacity (16, 32, 64 or 128)
+ * of the reference-algorithm kernel's instance (PLAIN_LU plans; the LU
+ * shapes of LU_Decomposition.cpp:7-72's port), 0 for MIXED -- with the
+ * measured ms of each candidate (0 for a candidate that does not apply, and
+ * none when only one candidate holds the plan), and the
  * z lanes per work-group chosen for each of the four transform passes
  * (R2C y, C2C x, C2C^-1 x, C2R y; 0 = the length's only shape) with the
  * winner's ms.  Any pointer may be NULL.  No counterpart in the reference. */
 int hz_plan_tuning(const hz_plan *plan, int *diag_tile_rows, double *diag_ms, int max_diag,
                    int *fft_lanes, double *fft_ms);
-/* The diagonalised mode kernel's candidate shapes, in the order of
- * hz_plan_tuning's diag_ms (*count = their number, at most `max` written),
- * and a way to pin one of them (it must hold one column's 2*ne element
- * rows).  For tests and tuning studies; no counterpart in the reference. */
+/* The mode kernel's candidate shapes (DIAG tile rows, or PLAIN_LU column
+ * capacities), in the order of hz_plan_tuning's diag_ms (*count = their
+ * number, at most `max` written), and a way to pin one of them (a DIAG tile
+ * must hold one column's 2*ne element rows).  For tests and tuning studies;
+ * no counterpart in the reference. */
 int hz_plan_diag_candidates(const hz_plan *plan, int *tile_rows, int max, int *count);
 int hz_plan_set_diag_tile(hz_plan *plan, int tile_rows);
 /* Runs the plan-time autotuning again (transform lane widths; DIAG tile
- * rows), e.g. once the GPU has reached its clocks under load: the plan's
- * device must be current.  Synchronises the device first (the tuners time on
- * the plan's own spectrum and face buffers, which a solve in flight would
- * share), allocates a scratch field for the transform timings, and leaves
- * the new choices and times for hz_plan_tuning.  Results of later solves are
- * unchanged: every DIAG tile height gives the same output bit for bit, and a
- * transform's lane width sets how many lines one work-group takes, not any
- * line's arithmetic (the radix split depends on the length only).  No
- * counterpart in the reference. */
+ * rows or PLAIN_LU instance), e.g. once the GPU has reached its clocks
+ * under load.  The plan's device must be current.  Synchronises the device
+ * first (the tuners time on the plan's own spectrum and face buffers, which
+ * a solve in flight would share), allocates a scratch field for the
+ * transform timings, clears the last solve's statuses and periodic
+ * closures, and leaves the new choices and times for hz_plan_tuning.
+ * Results of later solves are unchanged: every DIAG tile height and
+ * PLAIN_LU instance gives the same output bit for bit, and a transform's
+ * lane width sets how many lines one work-group takes, not any line's
+ * arithmetic (the radix split depends on the length only).  No counterpart
+ * in the reference. */
 int hz_plan_retune(hz_plan *plan);
 /* Periodic-z plans: the closure coefficient of the cyclic element-interface
  * system per real spectral column of the last solve (the reference's

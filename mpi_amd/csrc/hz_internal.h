@@ -86,8 +86,12 @@ struct LuModeArgs {
     double alpha_l, beta_l, alpha_r, beta_r;
     int *status;             // [modes]
     int periodic;            // z periodic: cyclic interface system, no face data
+    int cols;                // instance column capacity (lu_modes_shapes); 0 = the smallest
 };
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s);
+// Column capacities (16, 32, 64, 128) of the mode-kernel instances that hold
+// (nx, N-2, ne), smallest first; returns their number (at most `max` written).
+int lu_modes_shapes(int nx, int n, int ne, int *cols, int max);
 // Whether the plain-LU mode kernel has an instance for (nx, N-2, ne); sets
 // the error text when not.
 int lu_modes_check(int nx, int n, int ne);
@@ -241,6 +245,8 @@ struct hz_plan {
     double *d_eo_B1a = nullptr, *d_eo_B2a = nullptr;  // adjoint operands
     short *d_eo_M = nullptr, *d_eo_Ma = nullptr;      // first-order correction (bf16), forward and adjoint
     int diag_rt = 0;                                  // autotuned tile rows
+    int lu_cols = 0;                                  // autotuned PLAIN instance (0 = smallest)
+    std::vector<double> lu_cols_ms;                   // tuning times per LU instance
     int fft_w[4] = {0, 0, 0, 0};                      // autotuned lanes per pass
     double fft_w_ms[4] = {0, 0, 0, 0};                // winner's ms per pass
     std::vector<double> diag_rt_ms;                   // tuning times per choice

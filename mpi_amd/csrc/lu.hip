@@ -963,13 +963,37 @@ int lu_modes_check(int nx, int n, int ne)
     return HZ_OK;
 }
 
+int lu_modes_shapes(int nx, int n, int ne, int *cols, int max)
+{
+    // column capacity 4*CB of the four instances: 64 rows (RB = 1) up to 64
+    // columns, then 128 rows and columns, whose right-hand sides stop at 8
+    // per wave; an instance holds every smaller n (masked columns, padding rows)
+    static const int caps[] = {16, 32, 64, 128};
+    const int nr = (2 + rhs_per_element(nx) * ne + 3) / 4;
+    int k = 0;
+    for (int cap : caps) {
+        if (cap < n || nr > (cap == 128 ? 8 : 64)) continue;
+        if (k < max) cols[k] = cap;
+        ++k;
+    }
+    return k;
+}
+
 int launch_lu_modes(const LuModeArgs &a, int modes, hipStream_t s)
 {
     if (int st = lu_modes_check(a.nx, a.n, a.ne)) return st;
-    if (a.n <= 16) return launch_modes_nr<1, 4>(a, modes, s);
-    if (a.n <= 32) return launch_modes_nr<1, 8>(a, modes, s);
-    if (a.n <= 64) return launch_modes_nr<1, 16>(a, modes, s);
-    return launch_modes_nr<2, 32>(a, modes, s);
+    const int cols = a.cols ? a.cols : a.n <= 16 ? 16 : a.n <= 32 ? 32 : a.n <= 64 ? 64 : 128;
+    int c[4];
+    const int k = lu_modes_shapes(a.nx, a.n, a.ne, c, 4);
+    bool held = false;
+    for (int i = 0; i < k && i < 4; ++i) held = held || c[i] == cols;
+    if (!held) return set_error(HZ_ERR_ARG, "PLAIN_LU form: the mode-kernel shape does not hold this plan");
+    switch (cols) {
+    case 16: return launch_modes_nr<1, 4>(a, modes, s);
+    case 32: return launch_modes_nr<1, 8>(a, modes, s);
+    case 64: return launch_modes_nr<1, 16>(a, modes, s);
+    default: return launch_modes_nr<2, 32>(a, modes, s);
+    }
 }
 
 int launch_lu_factor(const LuFactorArgs &a, hipStream_t s)

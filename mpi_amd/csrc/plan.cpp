@@ -642,6 +642,95 @@ DiagArgs hz::diag_args(const hz_plan *p)
 
 namespace {
 
+// Kernel arguments of the reference-algorithm (PLAIN_LU) mode solve.
+LuModeArgs lu_args(const hz_plan *p, const double *op_diag = nullptr, const double *op_full = nullptr)
+{
+    LuModeArgs a{};
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.M = p->M;
+    a.K = p->K;
+    a.N = p->N;
+    a.n = p->n;
+    a.ne = p->ne;
+    a.DzzT = p->d_DzzT;
+    a.Dzz = p->d_Dzz;
+    a.Dz = p->d_Dz;
+    a.shift = p->d_shift;
+    a.op_diag = op_diag;
+    a.op_full = op_full;
+    a.spec_re = p->spec.re;
+    a.spec_im = p->spec.im;
+    a.xs = p->xs;
+    a.bc_re = p->bc.re;
+    a.bc_im = p->bc.im;
+    a.alpha_l = p->d.alpha_l;
+    a.beta_l = p->d.beta_l;
+    a.alpha_r = p->d.alpha_r;
+    a.beta_r = p->d.beta_r;
+    a.status = p->d_status;
+    a.periodic = p->d.periodic_z;
+    a.closure = p->d_closure;
+    a.cols = p->lu_cols;
+    return a;
+}
+
+// Plan-time autotuning of the reference-algorithm mode kernel: every
+// instance that holds the plan (column capacity 16, 32, 64 or 128) is timed
+// on noise (rounds over the candidates in turn, each one's best round of 3
+// back-to-back launches kept) and the fastest kept.  One candidate: nothing
+// to time.  The spectrum and face buffers are cleared afterwards.
+int tune_lu(hz_plan *p)
+{
+    int cols[4];
+    const int k = std::min(lu_modes_shapes(p->nx, p->n, p->ne, cols, 4), 4);
+    p->lu_cols = k ? cols[0] : 0;
+    p->lu_cols_ms.assign(k, 0.0);
+    if (k < 2) return HZ_OK;
+    const size_t nspec = 2 * (size_t)p->nx * p->xs, nbc = 4 * (size_t)p->nx * p->M;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = HZ_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        st = set_error(HZ_ERR_HIP, "tuning stream failed");
+    if (!st) st = launch_fill_noise(p->spec.re, nspec, 6, s);
+    if (!st) st = launch_fill_noise(p->bc.re, nbc, 7, s);
+    std::vector<double> ms_of(k, 1e300);
+    for (int round = 0; round < 3 && !st; ++round) {
+        for (int i = 0; i < k && !st; ++i) {
+            LuModeArgs a = lu_args(p);
+            a.cols = cols[i];
+            if (round == 0 && (st = launch_lu_modes(a, p->modes, s))) break;  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && !st; ++r) st = launch_lu_modes(a, p->modes, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) st = set_error(HZ_ERR_HIP, "tuning run failed");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms_of[i] = std::min(ms_of[i], ms / 3.0);
+        }
+    }
+    if (s && hipStreamSynchronize(s) != hipSuccess && !st) st = set_error(HZ_ERR_HIP, "tuning run failed");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (!st) {
+        double best = 1e300;
+        for (int i = 0; i < k; ++i) {
+            p->lu_cols_ms[i] = ms_of[i];
+            if (ms_of[i] < best) {
+                best = ms_of[i];
+                p->lu_cols = cols[i];
+            }
+        }
+    }
+    if (!st && (hipMemset(p->spec.re, 0, sizeof(double) * nspec) != hipSuccess ||
+                hipMemset(p->bc.re, 0, sizeof(double) * nbc) != hipSuccess))
+        st = set_error(HZ_ERR_HIP, "tuning buffer clear failed");
+    return st;
+}
+
 // Plan-time autotuning of the diagonalised mode kernel's tile height: each
 // choice is timed (one warm-up, three timed launches) on the plan's own
 // (zeroed) spectrum and the fastest kept.
@@ -864,12 +953,15 @@ int hz_plan_create(const hz_plan_desc *desc, hz_plan **plan)
     p->spec.im = p->spec.re + specN;
     p->bc.im = p->bc.re + bcN;
     HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
-    if ((d.form == HZ_FORM_PLAIN_LU && (st = lu_modes_check(p->nx, p->n, p->ne))) || (st = tune_fft(p)) ||
+    if ((d.form == HZ_FORM_PLAIN_LU && ((st = lu_modes_check(p->nx, p->n, p->ne)) || (st = tune_lu(p)))) ||
+        (st = tune_fft(p)) ||
         (d.form == HZ_FORM_DIAG && (st = tune_diag(p)))) {
         free_plan(p);
         return st;
     }
-    // after tuning: the tuned DIAG runs of a periodic plan write closures
+    // after tuning: the tuned mode-kernel runs write statuses, and closures
+    // for a periodic plan
+    HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
     if (p->d_closure) HZ_HIP(hipMemset(p->d_closure, 0, sizeof(double) * 2 * (size_t)p->nx * p->M));
     *plan = p;
     return HZ_OK;
@@ -944,33 +1036,7 @@ static int solve_impl(hz_plan *p, const double *f, const double *gl, const doubl
     rec.mark("fft_forward");
     if (st) return st;
     if (p->d.form == HZ_FORM_PLAIN_LU || op_diag || op_full) {
-        LuModeArgs a{};
-        a.nx = p->nx;
-        a.ny = p->ny;
-        a.M = p->M;
-        a.K = p->K;
-        a.N = p->N;
-        a.n = p->n;
-        a.ne = p->ne;
-        a.DzzT = p->d_DzzT;
-        a.Dzz = p->d_Dzz;
-        a.Dz = p->d_Dz;
-        a.shift = p->d_shift;
-        a.op_diag = op_diag;
-        a.op_full = op_full;
-        a.spec_re = p->spec.re;
-        a.spec_im = p->spec.im;
-        a.xs = p->xs;
-        a.bc_re = p->bc.re;
-        a.bc_im = p->bc.im;
-        a.alpha_l = p->d.alpha_l;
-        a.beta_l = p->d.beta_l;
-        a.alpha_r = p->d.alpha_r;
-        a.beta_r = p->d.beta_r;
-        a.status = p->d_status;
-        a.periodic = p->d.periodic_z;
-        a.closure = p->d_closure;
-        st = launch_lu_modes(a, p->modes, s);
+        st = launch_lu_modes(lu_args(p, op_diag, op_full), p->modes, s);
         rec.mark("lu_modes");
     } else if (p->d.form == HZ_FORM_DIAG) {
         DiagArgs a = diag_args(p);
@@ -1172,9 +1238,11 @@ int hz_plan_tuning(const hz_plan *p, int *diag_tile_rows, double *diag_ms, int m
                    double *fft_ms)
 {
     if (!p) return set_error(HZ_ERR_ARG, "null argument");
-    if (diag_tile_rows) *diag_tile_rows = p->diag_rt;
+    const bool lu = p->d.form == HZ_FORM_PLAIN_LU;
+    const std::vector<double> &ms = lu ? p->lu_cols_ms : p->diag_rt_ms;
+    if (diag_tile_rows) *diag_tile_rows = lu ? p->lu_cols : p->diag_rt;
     if (diag_ms)
-        for (int i = 0; i < (int)p->diag_rt_ms.size() && i < max_diag; ++i) diag_ms[i] = p->diag_rt_ms[i];
+        for (int i = 0; i < (int)ms.size() && i < max_diag; ++i) diag_ms[i] = ms[i];
     for (int pass = 0; pass < 4; ++pass) {
         if (fft_lanes) fft_lanes[pass] = p->fft_w[pass];
         if (fft_ms) fft_ms[pass] = p->fft_w_ms[pass];
@@ -1186,7 +1254,9 @@ int hz_plan_diag_candidates(const hz_plan *p, int *tile_rows, int max, int *coun
 {
     if (!p || !count) return set_error(HZ_ERR_ARG, "null argument");
     int c[8];
-    const int k = p->d.form == HZ_FORM_DIAG ? diag_tile_choices(p->NP, c, 8) : 0;
+    const int k = p->d.form == HZ_FORM_DIAG        ? diag_tile_choices(p->NP, c, 8)
+                  : p->d.form == HZ_FORM_PLAIN_LU ? std::min(lu_modes_shapes(p->nx, p->n, p->ne, c, 8), 8)
+                                                   : 0;
     for (int i = 0; i < k && i < max && tile_rows; ++i) tile_rows[i] = c[i];
     *count = k;
     return HZ_OK;
@@ -1195,8 +1265,17 @@ int hz_plan_diag_candidates(const hz_plan *p, int *tile_rows, int max, int *coun
 int hz_plan_set_diag_tile(hz_plan *p, int tile_rows)
 {
     if (!p) return set_error(HZ_ERR_ARG, "null argument");
-    if (p->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "not a DIAG plan");
     int c[8];
+    if (p->d.form == HZ_FORM_PLAIN_LU) {  // the LU instance, by its column capacity
+        const int k = std::min(lu_modes_shapes(p->nx, p->n, p->ne, c, 8), 8);
+        for (int i = 0; i < k; ++i)
+            if (c[i] == tile_rows) {
+                p->lu_cols = tile_rows;
+                return HZ_OK;
+            }
+        return set_error(HZ_ERR_ARG, "not a candidate mode-kernel shape for this plan");
+    }
+    if (p->d.form != HZ_FORM_DIAG) return set_error(HZ_ERR_UNSUPPORTED, "not a DIAG or PLAIN_LU plan");
     const int k = diag_tile_choices(p->NP, c, 8);
     bool known = false;
     for (int i = 0; i < k; ++i) known = known || c[i] == tile_rows;
@@ -1213,6 +1292,10 @@ int hz_plan_retune(hz_plan *p)
     HZ_HIP(hipDeviceSynchronize());
     int st = tune_fft(p);
     if (!st && p->d.form == HZ_FORM_DIAG) st = tune_diag(p);
+    if (!st && p->d.form == HZ_FORM_PLAIN_LU) st = tune_lu(p);
+    // the tuned runs of the mode kernels leave statuses and closures of noise
+    HZ_HIP(hipMemset(p->d_status, 0, sizeof(int) * p->modes));
+    if (p->d_closure) HZ_HIP(hipMemset(p->d_closure, 0, sizeof(double) * 2 * (size_t)p->nx * p->M));
     return st;
 }
 

@@ -218,14 +218,15 @@ class HelmholtzPlan:
                 "fft_lanes": list(lanes), "fft_ms": fms.tolist()}
 
     def diag_candidates(self):
-        """Tile shapes (rows per tile) the DIAG mode kernel offers for this plan."""
+        """Mode-kernel shapes offered for this plan: DIAG tile rows, or the
+        column capacities of the PLAIN_LU kernel's instances."""
         rows = (ctypes.c_int * 8)()
         k = ctypes.c_int(0)
         check(self.lib.hz_plan_diag_candidates(self.handle, rows, 8, ctypes.byref(k)))
         return list(rows)[:k.value]
 
     def set_diag_tile(self, rows):
-        """Pins the DIAG mode kernel's shape to one of diag_candidates()."""
+        """Pins the mode kernel's shape (DIAG or PLAIN_LU) to one of diag_candidates()."""
         check(self.lib.hz_plan_set_diag_tile(self.handle, int(rows)))
 
     def retune(self):

@@ -139,8 +139,9 @@ def test_diag_plan_autotune(cuda, N):
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
 def test_retune_other_forms(cuda, form):
-    """hz_plan_retune on the forms without a tile tuner re-times the
-    transform lane widths only; the solve stays the same bit for bit."""
+    """hz_plan_retune on a PLAIN plan re-times its LU instances and the
+    transform lane widths, on a MIXED plan the lane widths only; the solve
+    stays the same bit for bit."""
     from mpi_amd.solver import HelmholtzPlan
     plan = HelmholtzPlan(64, 32, 33, 1, lam=2.0, form=form)
     rng = np.random.default_rng(5)
@@ -149,8 +150,13 @@ def test_retune_other_forms(cuda, form):
     u = plan.solve(f, g, g).cpu().numpy()
     plan.retune()
     t = plan.tuning()
+    if form == "mixed":
+        assert t["diag_tile_rows"] == 0
+    else:  # N - 2 = 31: the LU instances of 32, 64 and 128 columns hold it
+        assert plan.diag_candidates() == [32, 64, 128] and t["diag_tile_rows"] in {32, 64, 128}
+        assert len(t["choice_ms"]) == 3
     # nx = 64: three lane widths for the x passes; ny = 32: one shape (0) for the y passes
-    assert t["diag_tile_rows"] == 0 and t["fft_lanes"][0] == 0 and t["fft_lanes"][3] == 0
+    assert t["fft_lanes"][0] == 0 and t["fft_lanes"][3] == 0
     assert t["fft_lanes"][1] in {8, 16, 32} and t["fft_lanes"][2] in {8, 16, 32}
     assert np.array_equal(plan.solve(f, g, g).cpu().numpy(), u)
     plan.close()

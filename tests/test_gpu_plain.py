@@ -196,3 +196,42 @@ def test_singular_operator_reported(cuda):
     with pytest.raises(HzError) as e:
         plan.status()
     assert e.value.code == HZ_ERR_SINGULAR
+
+
+@pytest.mark.parametrize("N,ne,robin", [(9, 1, False), (17, 1, False), (33, 2, True), (65, 1, True), (17, 8, True)])
+def test_plain_every_lu_instance(cuda, N, ne, robin):
+    """Every instance of the reference-algorithm mode kernel that holds the
+    plan (column capacity 16, 32, 64, 128, the plan-time tuner's candidates)
+    gives the same solve bit for bit, and that solve matches the oracle
+    within max(1e-12, 2 x its rounding spread)."""
+    import torch
+    from mpi_amd.solver import HelmholtzPlan
+    nx, ny, lam = 16, 8, 3.0
+    al, bl, ar, br = (0.6, 1.2, 0.3, 0.9) if robin else (0.0, 1.0, 0.0, 1.0)
+    plan = HelmholtzPlan(nx, ny, N, ne, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2, lam=lam,
+                         alpha_l=al, beta_l=bl, alpha_r=ar, beta_r=br, form="plain")
+    shapes = plan.diag_candidates()
+    t = plan.tuning()
+    assert shapes and t["diag_tile_rows"] in shapes
+    assert len(t["choice_ms"]) == (len(shapes) if len(shapes) > 1 else 0)
+    rng = np.random.default_rng(N + 10 * ne)
+    f = rng.standard_normal((nx, ny, plan.nz))
+    gl = rng.standard_normal((nx, ny))
+    gr = rng.standard_normal((nx, ny))
+    outs = []
+    for cols in shapes:
+        print(f"case N={N} ne={ne} robin={robin} LU instance {cols} columns", flush=True)
+        plan.set_diag_tile(cols)
+        outs.append(plan.solve(to_dev(f), to_dev(gl), to_dev(gr)))
+        plan.status()
+    for cols, u in zip(shapes[1:], outs[1:]):
+        assert torch.equal(u, outs[0]), f"instance {cols} differs from instance {shapes[0]}"
+    desc = po.make_desc(nx, ny, ne * N, N, size=1, xl=0, xr=2 * math.pi, yl=0, yr=3.0, zl=0, zr=2,
+                        alphal=al, betal=bl, alphar=ar, betar=br, lam=lam)
+    uo = po.solve(desc, f, gl, gr)
+    uf = po.solve(desc, f, gl, gr, fma=True)
+    bar = max(1e-12, 2.0 * float(np.abs(uf - uo).max() / np.abs(uo).max()))
+    err = float(np.abs(outs[0].cpu().numpy() - uo).max() / np.abs(uo).max())
+    print(f"  {len(shapes)} instances, rel err {err:.2e}, bar {bar:.2e}")
+    assert err <= bar
+    plan.close()
