@@ -109,5 +109,12 @@ def test_periodic_closure_matches_oracle(cuda, form):
     one = HelmholtzPlan(8, 8, 9, 1, lam=1.0, form=form, periodic_z=True)
     one.solve(dev(np.ones((8, 8, one.nz))))
     assert not one.periodic_closure().any()  # one element: no closure, as in the reference
+    # tuning again runs the mode kernels on noise: the last solve's closures
+    # are cleared, and the next solve gives them again bit for bit
+    plan.retune()
+    assert not plan.periodic_closure().any()
+    plan.status()
+    plan.solve(dev(f))
+    assert np.array_equal(plan.periodic_closure(), half)
     plan.close()
     one.close()
