@@ -354,3 +354,45 @@ def test_batch_with_mixed_plans(cuda):
         print(f"mixed problem {k}: rel err {e:.2e}, bar {bar:.2e}")
         assert e <= bar
     b.close()
+
+
+# one Dirichlet element of even N (the even/odd split with no centre
+# coordinate), padded sizes 32, 64, 32 and 128
+EVEN_SET = [(16, 32, 18, 1, 0.5, False), (64, 64, 66, 1, 10.0, False), (32, 16, 34, 1, 3.0, False),
+            (128, 32, 130, 1, 2.0, False)]
+
+
+def test_batch_even_element_sizes(cuda):
+    """DIAG and MIXED plans of even N in one batch: each problem equals its
+    single-plan solve (staged and merged launches, a graph replay) and the
+    oracle to the rounding-spread bar."""
+    from mpi_amd.solver import BatchSolver
+    forms = ["diag", "mixed", "mixed", "diag"]
+    plans, inputs, descs = make_set(EVEN_SET, forms)
+    fs = [dev(f) for f, _, _ in inputs]
+    gls = [dev(g) for _, g, _ in inputs]
+    grs = [dev(g) for _, _, g in inputs]
+    want = [p.solve(f, gl, gr).cpu().numpy() for p, f, gl, gr in zip(plans, fs, gls, grs)]
+    b = BatchSolver(plans)
+    for rep in range(2):  # autotune + capture, then a replay
+        us = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(us, want)):
+            e = rel(u.cpu().numpy(), w)
+            assert e <= 1e-13, f"rep {rep} problem {k} ({EVEN_SET[k]}): {e:.3e}"
+    for v in MERGED:
+        b.force_variant(v)
+        got = b.solve(fs, gls, grs)
+        b.status()
+        for k, (u, w) in enumerate(zip(got, want)):
+            e = rel(u.cpu().numpy(), w)
+            assert e <= 1e-13, f"merged variant {v} problem {k}: {e:.3e}"
+    for k, (f, gl, gr) in enumerate(inputs):
+        uo = po.solve(descs[k], f, gl, gr)
+        e = rel(us[k].cpu().numpy(), uo)
+        bar = max(1e-12, 2 * rel(po.solve(descs[k], f, gl, gr, fma=True), uo))
+        print(f"even problem {k} {EVEN_SET[k]} ({forms[k]}): rel err {e:.2e}, bar {bar:.2e}")
+        assert e <= bar
+    b.close()
+    for p in plans:
+        p.close()
